@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Throughput of the Fourier-projection hot path on one MI355X.
+
+A step renders every view of the workload once (direct sum over the Hermitian
+half-plane + C2R), with the particles already registered in HBM.  The metric
+is particle*frequency terms per second: N * 2*cx*cy terms per view.
+
+  python bench.py [--steps K] [--warmup W] [--workload lognormal_field] [--form auto|plain|...]
+
+Prints one JSON line (rank 0, one GPU).  --gpus above 1 exits: this build
+measures one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle·freq terms/s and views/s at rel-L2≤1e-5: MFMA split vs plain FP32"
+PEAKS = {  # MI355X published dense peaks
+    "hbm_GBps": 8000.0,
+    "fp32_TFLOPs": 157.3,
+    "bf16_TFLOPs": 2500.0,
+    "fp16_TFLOPs": 2500.0,
+}
+FLOPS_PER_TERM = 8.0  # one complex multiply-accumulate of w*Phi by exp(-i theta)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name: str, n: int | None):
+    from libmeshlessvis_amd import workloads as W
+
+    if name == "lognormal_field":
+        return W.lognormal_field(n or 4_194_304)
+    if name == "clustered_wendland":
+        return W.clustered_wendland(n or 262_144)
+    if name == "uniform_signed_gaussian":
+        return W.uniform_signed_gaussian(n or 1_048_576)
+    if name == "uniform_gaussian_4096":
+        return W.uniform_gaussian_4096()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def sample_indices(cutoff, rng, low=8, n_random=512):
+    """Stratified frequency sample: every index with |x| <= low and y <= low, plus n_random others.
+    Returns (idx, weight) so that sum(weight * |z[idx]|^2) estimates the sum over the whole grid."""
+    cx, cy = cutoff
+    total = 2 * cx * cy
+    x0 = np.arange(2 * cx)
+    x = np.where(x0 > cx, x0 - 2 * cx, x0)
+    low_idx = [y * 2 * cx + i for y in range(min(cy, low + 1)) for i in range(2 * cx) if abs(x[i]) <= low]
+    low_idx = np.array(low_idx, dtype=np.int64)
+    rest = np.setdiff1d(np.arange(total), low_idx)
+    pick = rng.choice(rest, size=min(n_random, rest.size), replace=False)
+    idx = np.concatenate([low_idx, pick])
+    weight = np.concatenate([np.ones(low_idx.size), np.full(pick.size, rest.size / max(1, pick.size))])
+    return idx, weight
+
+
+def rel_l2_sampled(F_flat, F64_at, idx, weight) -> float:
+    err = np.abs(F_flat[idx].astype(np.complex128) - F64_at) ** 2
+    ref = np.abs(F64_at) ** 2
+    return float(np.sqrt((weight * err).sum() / (weight * ref).sum()))
+
+
+def time_renders(cfgs, ds, steps, warmup):
+    for _ in range(warmup):
+        for c in cfgs:
+            c.render(ds)
+    for c in cfgs:
+        c.synchronize()
+        c.reset_timing()
+        c.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for c in cfgs:
+            c.render(ds)
+    for c in cfgs:
+        c.synchronize()
+    elapsed = time.perf_counter() - t0
+    launches, kernel_ms = 0, 0.0
+    for c in cfgs:
+        n, ms = c.kernel_timing()
+        launches += n
+        kernel_ms += ms
+        c.set_timing(False)
+    return elapsed, launches, kernel_ms
+
+
+def cpu_baseline(wl, threads: int):
+    """The FP32 restatement of the reference CPU direct sum, timed on a bounded sample of view 0."""
+    import oracle
+
+    vw = wl.views[0]
+    n_sub = min(wl.n, 262_144)
+    cy = wl.cutoff[1]
+    row_stride = max(1, cy // 16)
+    rows = len(range(0, cy, row_stride))
+    sub = (wl.positions[:n_sub], wl.weights[:n_sub], None if wl.radii is None else wl.radii[:n_sub], wl.basis)
+    t0 = time.perf_counter()
+    oracle.port_spectrum32([sub], wl.cutoff, wl.step, vw.u, vw.v, row_stride=row_stride, nthreads=threads)
+    dt = time.perf_counter() - t0
+    terms = n_sub * 2 * wl.cutoff[0] * rows
+    return {
+        "value": terms / dt,
+        "unit": "terms/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"view 0, {rows} of {cy} v-rows x {n_sub} of {wl.n} particles ({terms:.3e} terms, {dt:.1f} s)",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="lognormal_field")
+    ap.add_argument("--n", type=int, default=None, help="override the particle count")
+    ap.add_argument("--form", default="auto")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-accuracy", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (views would shard "
+              "across GPUs with no communication, see the design notes)", file=sys.stderr)
+        sys.exit(2)
+
+    from libmeshlessvis_amd import host
+    import oracle
+
+    t = time.perf_counter()
+    wl = make_workload(args.workload, args.n)
+    log(f"workload {wl.name}: N={wl.n} cutoff={wl.cutoff} samples={wl.samples} views={len(wl.views)} "
+        f"({time.perf_counter() - t:.1f} s to generate)")
+    form = "plain" if args.form == "auto" else args.form
+
+    ds = host.Dataset([wl.group()])
+    cfgs = [host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
+                        phi_mode=wl.phi_mode) for vw in wl.views]
+    cfgs[0].register(ds)
+    log(f"registered; timing form={form}")
+    elapsed, launches, kernel_ms = time_renders(cfgs, ds, args.steps, args.warmup)
+    views = args.steps * len(wl.views)
+    terms = views * wl.terms_per_view
+    value = terms / elapsed
+    log(f"{form}: {elapsed:.3f} s for {views} views -> {value:.3e} terms/s; kernel {kernel_ms / max(1, launches):.3f} ms/launch")
+
+    rel = None
+    if not args.no_accuracy:
+        rng = np.random.default_rng(0)
+        idx, weight = sample_indices(wl.cutoff, rng)
+        vw = wl.views[0]
+        cfgs[0].render(ds)
+        F = cfgs[0].spectrum().reshape(-1)
+        t = time.perf_counter()
+        F64 = oracle.spectrum64([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, wl.phi_mode, idx=idx)
+        rel = rel_l2_sampled(F, F64, idx, weight)
+        log(f"rel-L2 vs FP64 oracle (stratified sample of {idx.size} frequencies): {rel:.3e} "
+            f"({time.perf_counter() - t:.1f} s)")
+
+    avg_ms = kernel_ms / max(1, launches)
+    launch_terms = wl.terms_per_view
+    achieved_tflops = FLOPS_PER_TERM * launch_terms / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
+    if form == "plain":
+        bound, peak = "valu", PEAKS["fp32_TFLOPs"]
+    else:
+        bound, peak = "mfma", PEAKS["bf16_TFLOPs"]
+    roofline = {
+        "bound": bound,
+        "achieved": achieved_tflops,
+        "peak": peak,
+        "unit": "TFLOP/s",
+        "frac": (achieved_tflops / peak) if achieved_tflops else None,
+        "traffic": None,
+        "kernel": "direct_sum_kernel" if form == "plain" else "mfma_sum_kernel",
+        "kernel_ms_per_launch": avg_ms,
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, threads)
+        log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({cpu['sample']})")
+
+    for c in cfgs[1:]:
+        c.close()
+    cfgs[0].unregister(ds)
+    cfgs[0].close()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "terms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if form == "plain" else form,
+        "data": "synthetic",
+        "config": {
+            "workload": wl.name,
+            "particles": wl.n,
+            "cutoff": list(wl.cutoff),
+            "image": f"{wl.samples[0]}x{wl.samples[1]}",
+            "views_per_step": len(wl.views),
+            "basis": wl.basis,
+            "phi_mode": wl.phi_mode,
+            "form": form,
+        },
+        "views_per_s": views / elapsed,
+        "rel_l2_vs_fp64": rel,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * meshless_vis.h -- Fourier volume rendering of meshless datasets on MI355X
+ * (gfx950), C ABI of libmeshless_vis.so.
+ *
+ * Drop-in for the reference's include/meshless_vis.h: the VisConfig fields
+ * the reference declares keep their names, types and order, and every
+ * function keeps its name and signature, except where noted below.  A render
+ * is: direct nonuniform sum of the particles' Fourier transform over the
+ * Hermitian half of the view's slice plane (hand-written HIP kernels), then an
+ * unnormalised 2-D C2R inverse FFT (hand-written HIP Stockham kernels) into
+ * _d_image (float[Nx*Ny], row-major, first index along u_axis).
+ *
+ * Differences from the reference, all deliberate:
+ *  - _plan is an opaque library handle (the reference holds a cufftHandle or
+ *    fftwf_plan there, meshless_vis.h:129-133).
+ *  - errors never abort the process: every entry point records a code and a
+ *    message, read with vis_get_last_error_code()/vis_get_last_error().  The
+ *    reference aborts through cutil macros (meshless_vis.cu:141-146).
+ *  - vis_fourier_volume_rendering validates the config with the
+ *    vis_config_check rules and renders nothing (error VIS_ERROR_INVALID_CONFIG)
+ *    when they fail; the reference silently skips an unsupported
+ *    block_length (fourier_transform.cu:167-171).
+ *  - vis_config_manual_d_image takes float*, as the reference's definitions
+ *    do (meshless_vis.cu:244); its header said float2*.
+ *  - vis_opengl_fourier_volume_rendering has no GL interop on a headless
+ *    build: it records VIS_ERROR_UNSUPPORTED and does nothing.
+ *  - fields after _d_freq_image_arranged are additive (form, phi_mode, ...).
+ */
+#ifndef MESHLESS_VISh
+#define MESHLESS_VISh
+
+#include "meshless.h"
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C"
+{
+#endif
+
+/* Additive: which implementation of the direct sum runs. */
+enum VisForm
+{
+	VIS_FORM_PLAIN = 0,            /* FP32 direct sum, fixed-point (FP64-reduced) phase     */
+	VIS_FORM_MFMA_F16 = 1,         /* one FP16 MFMA complex GEMM, FP32 accumulate           */
+	VIS_FORM_MFMA_SPLIT_BF16 = 2,  /* hi/lo BF16 operands, 3 products, FP32 accumulate      */
+	VIS_FORM_MFMA_SPLIT_F16 = 3    /* hi/lo FP16 operands, 3 products, FP32 accumulate      */
+};
+
+/* Additive: which Phi (Fourier transform of the basis) is evaluated. */
+enum VisPhiMode
+{
+	/* The reference's formulas (fourier_transform_cpu.cpp:37-66): quadratic fits
+	   below r = 0.06 (SPH) / 0.24 (Wendland), and exp(+pi r^2) for GAUSSIAN. */
+	VIS_PHI_REFERENCE = 0,
+	/* Exact transforms: series about 0 then closed forms, exp(-pi r^2) for GAUSSIAN
+	   (the transform of exp(-pi |x|^2)). */
+	VIS_PHI_ANALYTIC = 1
+};
+
+/* Additive: error codes of vis_get_last_error_code(). */
+enum VisError
+{
+	VIS_SUCCESS = 0,
+	VIS_ERROR_INVALID_VALUE = 1,
+	VIS_ERROR_INVALID_CONFIG = 2,
+	VIS_ERROR_HIP = 3,
+	VIS_ERROR_OUT_OF_MEMORY = 4,
+	VIS_ERROR_UNSUPPORTED = 5,
+	VIS_ERROR_NOT_REGISTERED = 6
+};
+
+typedef struct VisPlan VisPlan; /* opaque, owned by the library */
+
+/* Replaces include/meshless_vis.h:39-79. */
+typedef struct
+{
+	// variables prefixed with an underscore are private and are only changed by vis_config_* functions
+
+	float2 step_size;
+	int2 _cutoff_frequency;
+	float3 u_axis;
+	float3 v_axis;
+
+	bool cull_fully_aliased_terms;   /* kept for layout; never read (as in the reference) */
+
+	bool _automatic_d_image;
+
+	VisPlan* _plan;
+
+	int2 _number_of_samples;
+	int block_length;
+	float _scale;
+	float* _d_image;
+
+	float2* _d_freq_image;           /* device, P * 2*cx*cy complex, [p][y][x0] */
+
+	int _number_of_partial_sums;
+
+	float2* _d_freq_image_arranged;  /* device, Nx * (Ny/2+1) complex: FFT work buffer */
+
+	/* ---- additive fields (no counterpart in the reference) ---- */
+	int form;                           /* enum VisForm, default VIS_FORM_PLAIN        */
+	int phi_mode;                       /* enum VisPhiMode, default VIS_PHI_REFERENCE  */
+	int number_of_h_bins;               /* MFMA form, SPH/Wendland radii: 0 = auto     */
+	int number_of_interpolation_nodes;  /* MFMA form, SPH/Wendland radii: nodes per bin, 0 = auto */
+} VisConfig;
+
+/* meshless_vis.h:81 / meshless_vis.cu:129-176.  Returns 0 (and sets the last error) on failure. */
+VisConfig* vis_config_create(bool automatic_d_image, float2 step_size, int2 cutoff_frequency, float3 u_axis, float3 v_axis, int2 number_of_samples, int block_length, int number_of_partial_sums);
+/* meshless_vis.h:82 / meshless_vis.cu:178-189: step 0.1, cutoff 128, 512x512, block 256, 1 partial sum. */
+VisConfig* vis_config_get_default();
+/* meshless_vis.h:83 / meshless_vis.cu:191-210 (wavefront size is 64 on gfx950). */
+bool vis_config_check(VisConfig* vis_config);
+/* meshless_vis.h:84 / meshless_vis.cu:244-247 */
+void vis_config_manual_d_image(VisConfig* vis_config, float* d_image);
+/* meshless_vis.h:85-87 / meshless_vis.cu:212-242 */
+void vis_config_change_number_of_samples(VisConfig* vis_config, int2 number_of_samples);
+void vis_config_change_cutoff_frequency(VisConfig* vis_config, int2 cutoff_frequency);
+void vis_config_change_number_of_partial_sums(VisConfig* vis_config, int number_of_partial_sums);
+/* meshless_vis.h:88 / meshless_vis.cu:249-256 */
+void vis_config_destroy(VisConfig* vis_config);
+/* meshless_vis.h:89 / meshless_vis.cu:258-262: _scale = step_size.x * step_size.y */
+void vis_config_compute_scale(VisConfig* vis_config);
+/* meshless_vis.h:90-91 / meshless_vis.cu:264-299: H2D copy, zero padding to a multiple of P*block_length */
+void vis_register_meshless_dataset(VisConfig* vis_config, MeshlessDataset* meshless_dataset);
+void vis_unregister_meshless_dataset(VisConfig* vis_config, MeshlessDataset* meshless_dataset);
+/* meshless_vis.h:92 / meshless_vis.cu:357-364: unsupported here (records VIS_ERROR_UNSUPPORTED). */
+void vis_opengl_fourier_volume_rendering(MeshlessDataset* meshless_dataset, VisConfig* vis_config, unsigned int buffer_object);
+/* meshless_vis.h:93 / meshless_vis.cu:334-350: asynchronous on the config's stream. */
+void vis_fourier_volume_rendering(MeshlessDataset* meshless_dataset, VisConfig* vis_config);
+/* meshless_vis.h:94 / meshless_vis.cu:352-355: synchronising D2H copy of Nx*Ny floats. */
+void vis_copy_to_host(VisConfig* vis_config, float* h_image);
+
+/* The reference's internal seam (fourier_transform.h:33, fourier_transform.cu:161-181):
+   spectrum only, into _d_freq_image slot 0, no FFT. */
+void fourier_transform(MeshlessDataset* meshless_dataset, VisConfig* vis_config);
+
+/* ---- additive entry points (no counterpart in the reference) ---- */
+
+/* Stream every later call on this config runs on (default: the null stream). */
+void vis_set_stream(VisConfig* vis_config, hipStream_t stream);
+hipStream_t vis_get_stream(VisConfig* vis_config);
+/* Wait for the config's stream. */
+void vis_synchronize(VisConfig* vis_config);
+/* Synchronising D2H copy of the 2*cx*cy complex spectrum (slot 0, layout [y][x0]). */
+void vis_copy_spectrum_to_host(VisConfig* vis_config, float2* h_spectrum);
+
+/* Render many (dataset, config) pairs; all configs must share one stream.
+   Equivalent to calling vis_fourier_volume_rendering on each pair in order. */
+void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, VisConfig** vis_configs, int number_of_renders);
+
+/* Kernel timing with HIP events on the config's stream around the direct-sum
+   kernel (the dominant kernel of a render).  Off by default. */
+void vis_set_kernel_timing(VisConfig* vis_config, int enable);
+/* Returns the number of timed launches since the last reset and their summed
+   duration in milliseconds (synchronises the config's stream). */
+int vis_get_kernel_timing(VisConfig* vis_config, double* total_ms);
+void vis_reset_kernel_timing(VisConfig* vis_config);
+
+/* Last error of the calling thread (VIS_SUCCESS / "" when none). */
+int vis_get_last_error_code(void);
+const char* vis_get_last_error(void);
+void vis_clear_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

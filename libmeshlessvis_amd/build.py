@@ -1,0 +1,87 @@
+"""Build libmeshless_vis.so (HIP, gfx950) in-tree with hipcc.
+
+    python -m libmeshlessvis_amd.build          # or build() from Python
+
+Every source under csrc/ is compiled for gfx950 into _build/ at the repository
+root, then linked into libmeshlessvis_amd/libmeshless_vis.so.  The FP32
+packed-math feature is switched off for device code (the gfx950 packed FP32
+instructions are kept out of every kernel).
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+BUILD = ROOT / "_build" / "libmeshless_vis"
+LIB = PKG / "libmeshless_vis.so"
+ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
+
+SOURCES = ["vis_runtime.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip"]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (set HIPCC)")
+
+
+def _common_flags() -> list[str]:
+    return [
+        f"--offload-arch={ARCH}",
+        "-O3",
+        "-fPIC",
+        "-std=c++17",
+        "-Wall",
+        "-Wno-unused-result",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        f"-I{ROOT / 'include'}",
+    ]
+
+
+def _compile(src: Path, obj: Path) -> None:
+    cmd = [_hipcc(), *_common_flags()]
+    if src.suffix == ".cpp":
+        cmd += ["-x", "hip"]
+    cmd += ["-c", str(src), "-o", str(obj)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    noise = [l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()]
+    if noise:
+        sys.stderr.write(f"[build] {src.name}:\n" + "\n".join(noise) + "\n")
+
+
+def build(verbose: bool = True) -> Path:
+    """Compile every HIP source for gfx950 and link libmeshless_vis.so; returns its path."""
+    BUILD.mkdir(parents=True, exist_ok=True)
+    objs = []
+    jobs = []
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        for name in SOURCES:
+            src = CSRC / name
+            obj = BUILD / (name + ".o")
+            objs.append(obj)
+            jobs.append(ex.submit(_compile, src, obj))
+        for j in jobs:
+            j.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"[build] {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build()

@@ -1,0 +1,90 @@
+// Internal declarations shared by the host runtime and the HIP kernel files of
+// libmeshless_vis.so.  Nothing here is part of the C ABI.
+#pragma once
+
+#include "../../include/meshless_vis.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <mutex>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace mv {
+
+// ---------------------------------------------------------------- errors ---
+void set_error(int code, const std::string& msg);
+void clear_error();
+bool hip_ok(hipError_t e, const char* what);  // records VIS_ERROR_HIP / OOM on failure
+
+// ------------------------------------------------------- particle packets ---
+// Per-view, per-particle record written by the prep kernel and streamed by the
+// direct-sum kernels: the two fixed-point phase steps (revolutions * 2^32,
+// reduced mod 1 in FP64), the weight and the radius.
+struct alignas(16) Packet {
+  uint32_t pu;  // frac(step.x * dot(u_axis, position)) * 2^32
+  uint32_t pv;  // frac(step.y * dot(v_axis, position)) * 2^32
+  float w;      // weight
+  float h;      // radius r0 (1 when the group has no radii)
+};
+
+// Registration facts about a group, keyed by its device constraint pointer.
+struct GroupInfo {
+  int n = 0;                 // real particle count (number_of_terms)
+  int padded = 0;            // d_number_of_terms
+  bool has_radii = false;
+  bool uniform_radius = false;  // every radius equal (or no radii)
+  float radius = 1.0f;          // the common radius when uniform_radius
+  float h_min = 1.0f, h_max = 1.0f;
+  float w_absmax = 0.0f;
+};
+void registry_put(const void* key, const GroupInfo& info);
+bool registry_get(const void* key, GroupInfo* info);
+void registry_erase(const void* key);
+
+// ------------------------------------------------------------ device buffers
+// Grow-only device scratch owned by a plan.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  bool reserve(size_t need);  // false (error recorded) on failure
+  void release();
+  template <class T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+// Start/stop events of one timed kernel launch.
+using EventPair = std::pair<hipEvent_t, hipEvent_t>;
+
+// ----------------------------------------------------- view description ---
+// Everything a kernel needs to know about one view, passed by value.
+struct ViewParams {
+  int cx, cy;          // cutoff frequency: x in [-cx+1, cx], y in [0, cy)
+  float sx, sy;        // step_size
+  int basis;           // BasisFunctionId
+  int phi_mode;        // VisPhiMode
+};
+
+// ------------------------------------------------------ kernel launchers ---
+// prep.hip
+hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
+                               double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s);
+
+// direct_sum.hip: plain FP32 direct sum over [first, first+count) packets.
+//   phi_kind: 0 = uniform radius (Phi hoisted per frequency), 1 = per-particle radius.
+//   partial: [ksplit][2cx*cy] double2, slot k holds the sum of split k (unscaled, +i sin).
+hipError_t launch_direct_sum(const Packet* packets, int n_padded, int ksplit, int uniform, float radius,
+                             const ViewParams& vp, double2* partial, hipStream_t s);
+// Sum the ksplit partials (in order) into acc (assign when first), FP64.
+hipError_t launch_reduce_partials(const double2* partial, int ksplit, int nfreq, int first, double2* acc,
+                                  hipStream_t s);
+// F[i] = (scale*re, -scale*im) as float2 (slot 0 of _d_freq_image).
+hipError_t launch_finalize_spectrum(const double2* acc, int nfreq, float scale, float2* F, hipStream_t s);
+
+// fft.hip: 2-D C2R of the half-plane spectrum F[cy][2cx] into image[Nx][Ny];
+// work is the Nx*(Ny/2+1) complex buffer (_d_freq_image_arranged).
+hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const float2* tw_x, const float2* tw_y,
+                      float2* work, float* image, hipStream_t s);
+
+}  // namespace mv

@@ -1,0 +1,57 @@
+// Per-view particle preparation: one pass over the registered AoS particles
+// that writes the 16-byte packets the direct-sum kernels stream.
+//
+// The phase of particle i at frequency (x, y) of the slice plane is, in
+// revolutions, x*a_i + y*b_i with a_i = step.x*dot(u_axis, p_i) and
+// b_i = step.y*dot(v_axis, p_i) (the reference forms 2*pi*dot(k, p) in FP32,
+// fourier_transform_cpu.cpp:300).  Only the fractional parts of a_i and b_i
+// matter because x and y are integers, so they are reduced here in FP64 and
+// stored as unsigned 32-bit fixed point; the kernels then form x*a + y*b
+// modulo one revolution exactly, by wrapping integer arithmetic.  The
+// quantisation error is at most (|x|+|y|) * 2^-33 revolutions.
+#include "internal.h"
+
+namespace mv {
+
+__device__ __forceinline__ uint32_t to_fixed_rev(double a) {
+  const double f = a - floor(a);                       // [0, 1)
+  const double scaled = f * 4294967296.0 + 0.5;        // round to nearest
+  return static_cast<uint32_t>(static_cast<uint64_t>(scaled));  // 2^32 wraps to 0
+}
+
+__global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __restrict__ c,
+                                                           const float* __restrict__ radii, int n, int n_padded,
+                                                           double sx, double sy, double3 u, double3 v,
+                                                           Packet* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_padded) return;
+  Packet p;
+  if (i < n) {
+    const float4 q = reinterpret_cast<const float4*>(c)[i];
+    const double px = q.x, py = q.y, pz = q.z;
+    const double du = fma(u.x, px, fma(u.y, py, u.z * pz));
+    const double dv = fma(v.x, px, fma(v.y, py, v.z * pz));
+    p.pu = to_fixed_rev(sx * du);
+    p.pv = to_fixed_rev(sy * dv);
+    p.w = q.w;
+    p.h = radii ? radii[i] : 1.0f;
+  } else {
+    p.pu = 0u;
+    p.pv = 0u;
+    p.w = 0.0f;
+    p.h = 0.0f;
+  }
+  out[i] = p;
+}
+
+hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
+                               double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s) {
+  if (n_padded <= 0) return hipSuccess;
+  const int block = 256;
+  const int grid = (n_padded + block - 1) / block;
+  hipLaunchKernelGGL(prep_packets_kernel, dim3(grid), dim3(block), 0, s, d_constraints, d_radii, n, n_padded, sx,
+                     sy, u, v, out);
+  return hipGetLastError();
+}
+
+}  // namespace mv

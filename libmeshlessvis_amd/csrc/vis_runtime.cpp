@@ -1,0 +1,624 @@
+// Host runtime of libmeshless_vis.so: VisConfig lifecycle, dataset
+// registration, the per-view render pipeline and the additive entry points.
+//
+// Reference counterparts: meshless_vis/meshless_vis.cu:129-355 (GPU runtime)
+// and meshless_vis/fourier_transform.cu:161-181 (dispatch).  The reference
+// aborts on any CUDA error; here every entry point records an error code and
+// message instead (vis_get_last_error_code / vis_get_last_error).
+#include "internal.h"
+#include "mfma.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <unordered_map>
+
+namespace mv {
+
+// ---------------------------------------------------------------- errors ---
+static thread_local int g_err_code = VIS_SUCCESS;
+static thread_local std::string g_err_msg;
+
+void set_error(int code, const std::string& msg) {
+  g_err_code = code;
+  g_err_msg = msg;
+}
+void clear_error() {
+  g_err_code = VIS_SUCCESS;
+  g_err_msg.clear();
+}
+bool hip_ok(hipError_t e, const char* what) {
+  if (e == hipSuccess) return true;
+  const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? VIS_ERROR_OUT_OF_MEMORY : VIS_ERROR_HIP;
+  set_error(code, std::string(what) + ": " + hipGetErrorString(e));
+  (void)hipGetLastError();  // clear the sticky per-thread error of a failed launch
+  return false;
+}
+
+// -------------------------------------------------------------- registry ---
+static std::mutex g_reg_mu;
+static std::unordered_map<const void*, GroupInfo>& reg() {
+  static std::unordered_map<const void*, GroupInfo> m;
+  return m;
+}
+void registry_put(const void* key, const GroupInfo& info) {
+  std::lock_guard<std::mutex> lk(g_reg_mu);
+  reg()[key] = info;
+}
+bool registry_get(const void* key, GroupInfo* info) {
+  std::lock_guard<std::mutex> lk(g_reg_mu);
+  auto it = reg().find(key);
+  if (it == reg().end()) return false;
+  *info = it->second;
+  return true;
+}
+void registry_erase(const void* key) {
+  std::lock_guard<std::mutex> lk(g_reg_mu);
+  reg().erase(key);
+}
+
+// -------------------------------------------------------------- DevBuf ---
+bool DevBuf::reserve(size_t need) {
+  if (need <= bytes && ptr) return true;
+  release();
+  if (need == 0) return true;
+  void* p = nullptr;
+  if (!hip_ok(hipMalloc(&p, need), "hipMalloc(workspace)")) return false;
+  ptr = p;
+  bytes = need;
+  return true;
+}
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+}  // namespace mv
+
+// ------------------------------------------------------------------ plan ---
+struct VisPlan {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int nx = 0, ny = 0;
+  float2* tw_x = nullptr;
+  float2* tw_y = nullptr;
+  mv::DevBuf acc, partial, packets;
+  mv::MfmaWorkspace mfma;
+  bool timing = false;
+  std::vector<mv::EventPair> events;
+  size_t events_used = 0;
+};
+
+namespace mv {
+
+static bool make_twiddles(int n, float2** out) {
+  if (*out) (void)hipFree(*out);
+  *out = nullptr;
+  if (n <= 0) return true;
+  std::vector<float2> h(n);
+  for (int k = 0; k < n; ++k) {
+    const double a = 2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
+    h[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+  }
+  if (!hip_ok(hipMalloc(reinterpret_cast<void**>(out), sizeof(float2) * n), "hipMalloc(twiddles)")) return false;
+  return hip_ok(hipMemcpy(*out, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
+}
+
+static bool plan_set_samples(VisPlan* p, int nx, int ny) {
+  p->nx = nx;
+  p->ny = ny;
+  return make_twiddles(nx, &p->tw_x) && make_twiddles(ny, &p->tw_y);
+}
+
+static void plan_destroy(VisPlan* p) {
+  if (!p) return;
+  if (p->tw_x) (void)hipFree(p->tw_x);
+  if (p->tw_y) (void)hipFree(p->tw_y);
+  p->acc.release();
+  p->partial.release();
+  p->packets.release();
+  p->mfma.release();
+  for (auto& e : p->events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  delete p;
+}
+
+static size_t spectrum_count(const VisConfig* c) {
+  return static_cast<size_t>(2) * c->_cutoff_frequency.x * c->_cutoff_frequency.y;
+}
+
+static bool alloc_freq_image(VisConfig* c) {
+  if (c->_d_freq_image) (void)hipFree(c->_d_freq_image);
+  c->_d_freq_image = nullptr;
+  const size_t n = spectrum_count(c) * c->_number_of_partial_sums;
+  if (n == 0) return true;
+  return hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_freq_image), sizeof(float2) * n), "hipMalloc(_d_freq_image)");
+}
+
+static bool alloc_samples(VisConfig* c) {
+  if (c->_d_freq_image_arranged) (void)hipFree(c->_d_freq_image_arranged);
+  c->_d_freq_image_arranged = nullptr;
+  const size_t na = static_cast<size_t>(c->_number_of_samples.x) * (c->_number_of_samples.y / 2 + 1);
+  if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_freq_image_arranged), sizeof(float2) * std::max<size_t>(na, 1)),
+              "hipMalloc(_d_freq_image_arranged)"))
+    return false;
+  if (c->_automatic_d_image) {
+    if (c->_d_image) (void)hipFree(c->_d_image);
+    c->_d_image = nullptr;
+    const size_t ni = static_cast<size_t>(c->_number_of_samples.x) * c->_number_of_samples.y;
+    if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_image), sizeof(float) * std::max<size_t>(ni, 1)),
+                "hipMalloc(_d_image)"))
+      return false;
+  }
+  return plan_set_samples(c->_plan, c->_number_of_samples.x, c->_number_of_samples.y);
+}
+
+struct DeviceLimits {
+  int warp = 64, max_threads = 1024, max_dim0 = 1024;
+};
+static bool device_limits(DeviceLimits* out) {
+  static std::mutex mu;
+  static std::unordered_map<int, DeviceLimits> cache;
+  int dev = 0;
+  if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return false;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it == cache.end()) {
+    hipDeviceProp_t prop;
+    if (!hip_ok(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties")) return false;
+    DeviceLimits l;
+    l.warp = prop.warpSize;
+    l.max_threads = prop.maxThreadsPerBlock;
+    l.max_dim0 = prop.maxThreadsDim[0];
+    it = cache.emplace(dev, l).first;
+  }
+  *out = it->second;
+  return true;
+}
+
+// The vis_config_check rules (meshless_vis.cu:191-210) plus this build's limits.
+static bool config_valid(const VisConfig* c, std::string* why) {
+  DeviceLimits lim;
+  if (!device_limits(&lim)) {
+    *why = "cannot query device limits";
+    return false;
+  }
+  const int bl = c->block_length;
+  if (bl < 64) { *why = "block_length < 64"; return false; }
+  if (bl < lim.warp) { *why = "block_length < wavefront size"; return false; }
+  if (bl > lim.max_threads || bl > lim.max_dim0) { *why = "block_length > max threads per block"; return false; }
+  if ((2 * c->_cutoff_frequency.x * c->_cutoff_frequency.y) % bl) { *why = "2*cutoff.x*cutoff.y not a multiple of block_length"; return false; }
+  if (2 * c->_cutoff_frequency.x > c->_number_of_samples.x) { *why = "2*cutoff.x > number_of_samples.x"; return false; }
+  if (2 * c->_cutoff_frequency.y > c->_number_of_samples.y) { *why = "2*cutoff.y > number_of_samples.y"; return false; }
+  if (c->_number_of_partial_sums < 1) { *why = "number_of_partial_sums < 1"; return false; }
+  // Limits of this build (not in the reference).
+  if (c->_cutoff_frequency.x < 0 || c->_cutoff_frequency.y < 0) { *why = "negative cutoff"; return false; }
+  if (c->_number_of_samples.x > 4096 || c->_number_of_samples.y > 4096) { *why = "number_of_samples above 4096"; return false; }
+  if (c->form < VIS_FORM_PLAIN || c->form > VIS_FORM_MFMA_SPLIT_F16) { *why = "unknown form"; return false; }
+  if (c->phi_mode != VIS_PHI_REFERENCE && c->phi_mode != VIS_PHI_ANALYTIC) { *why = "unknown phi_mode"; return false; }
+  return true;
+}
+
+static EventPair* next_events(VisPlan* p) {
+  if (p->events_used == p->events.size()) {
+    hipEvent_t a, b;
+    if (!hip_ok(hipEventCreate(&a), "hipEventCreate")) return nullptr;
+    if (!hip_ok(hipEventCreate(&b), "hipEventCreate")) {
+      (void)hipEventDestroy(a);
+      return nullptr;
+    }
+    p->events.emplace_back(a, b);
+  }
+  return &p->events[p->events_used++];
+}
+
+// Number of particle splits for the plain kernel: enough workgroups to fill
+// the chip several times over, at least one 256-particle tile per split.
+static int choose_ksplit(int ntiles, int freq_tiles) {
+  const int target = 2048;
+  int k = (target + freq_tiles - 1) / std::max(1, freq_tiles);
+  k = std::min(k, std::max(1, ntiles));
+  return std::max(1, k);
+}
+
+static bool group_info_for(const Group& g, GroupInfo* info) {
+  if (registry_get(g.d_constraints, info)) return true;
+  // Registered by other means: assume per-particle radii when present.
+  info->n = g.number_of_terms;
+  info->padded = g.d_number_of_terms;
+  info->has_radii = g.d_radii != nullptr;
+  info->uniform_radius = g.d_radii == nullptr;
+  info->radius = 1.0f;
+  return true;
+}
+
+// The spectrum of all groups into _d_freq_image slot 0.
+static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
+  VisPlan* p = c->_plan;
+  const int nfreq = static_cast<int>(spectrum_count(c));
+  if (nfreq == 0) return true;
+  ViewParams vp;
+  vp.cx = c->_cutoff_frequency.x;
+  vp.cy = c->_cutoff_frequency.y;
+  vp.sx = c->step_size.x;
+  vp.sy = c->step_size.y;
+  vp.phi_mode = c->phi_mode;
+  const double3 u = make_double3(c->u_axis.x, c->u_axis.y, c->u_axis.z);
+  const double3 v = make_double3(c->v_axis.x, c->v_axis.y, c->v_axis.z);
+
+  if (!p->acc.reserve(sizeof(double2) * nfreq)) return false;
+  for (int gi = 0; gi < ds->number_of_groups; ++gi) {
+    const Group& g = ds->groups[gi];
+    if (g.number_of_terms > 0 && g.d_constraints == nullptr) {
+      set_error(VIS_ERROR_NOT_REGISTERED, "group " + std::to_string(gi) + " is not registered");
+      return false;
+    }
+    GroupInfo info;
+    group_info_for(g, &info);
+    const int n = std::max(0, std::min(info.n, g.d_number_of_terms > 0 ? g.d_number_of_terms : info.n));
+    vp.basis = g.basis_function_id;
+    if (vp.basis < SPH || vp.basis > WENDLAND_D3_C2) {
+      set_error(VIS_ERROR_INVALID_VALUE, "unknown basis_function_id");
+      return false;
+    }
+    if (c->form != VIS_FORM_PLAIN) {
+      if (!mfma_group_spectrum(g, info, n, vp, u, v, c, gi == 0, p->stream, &p->mfma, p->acc.as<double2>(),
+                               p->timing ? next_events(p) : nullptr))
+        return false;
+      continue;
+    }
+    const int n_padded = ((n + 255) / 256) * 256;
+    if (!p->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
+    if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
+                                    p->packets.as<Packet>(), p->stream),
+                "prep_packets launch"))
+      return false;
+    const int ntiles = n_padded / 256;
+    const int freq_tiles = ((2 * vp.cx + 63) / 64) * ((vp.cy + 31) / 32);
+    const int ksplit = choose_ksplit(ntiles, freq_tiles);
+    if (!p->partial.reserve(sizeof(double2) * static_cast<size_t>(ksplit) * nfreq)) return false;
+    EventPair* ev = p->timing ? next_events(p) : nullptr;
+    if (ev && !hip_ok(hipEventRecord(ev->first, p->stream), "hipEventRecord")) return false;
+    const bool uniform = info.uniform_radius;
+    const float radius = info.has_radii ? info.radius : 1.0f;
+    if (!hip_ok(launch_direct_sum(p->packets.as<Packet>(), n_padded, ksplit, uniform ? 1 : 0, radius, vp,
+                                  p->partial.as<double2>(), p->stream),
+                "direct_sum launch"))
+      return false;
+    if (ev && !hip_ok(hipEventRecord(ev->second, p->stream), "hipEventRecord")) return false;
+    if (!hip_ok(launch_reduce_partials(p->partial.as<double2>(), ksplit, nfreq, gi == 0 ? 1 : 0, p->acc.as<double2>(),
+                                       p->stream),
+                "reduce_partials launch"))
+      return false;
+  }
+  return hip_ok(launch_finalize_spectrum(p->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, p->stream),
+                "finalize launch");
+}
+
+}  // namespace mv
+
+using namespace mv;
+
+extern "C" {
+
+VisConfig* vis_config_create(bool automatic_d_image, float2 step_size, int2 cutoff_frequency, float3 u_axis,
+                             float3 v_axis, int2 number_of_samples, int block_length, int number_of_partial_sums) {
+  clear_error();
+  if (cutoff_frequency.x < 0 || cutoff_frequency.y < 0 || number_of_samples.x <= 0 || number_of_samples.y <= 0 ||
+      number_of_partial_sums < 1) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_config_create: negative cutoff, non-positive samples or partial sums < 1");
+    return nullptr;
+  }
+  VisConfig* c = static_cast<VisConfig*>(std::calloc(1, sizeof(VisConfig)));
+  if (!c) {
+    set_error(VIS_ERROR_OUT_OF_MEMORY, "vis_config_create: host allocation failed");
+    return nullptr;
+  }
+  c->step_size = step_size;
+  c->u_axis = u_axis;
+  c->v_axis = v_axis;
+  c->cull_fully_aliased_terms = false;
+  c->_number_of_samples = number_of_samples;
+  c->_cutoff_frequency = cutoff_frequency;
+  c->block_length = block_length;
+  c->_number_of_partial_sums = number_of_partial_sums;
+  c->_automatic_d_image = automatic_d_image;
+  c->form = VIS_FORM_PLAIN;
+  c->phi_mode = VIS_PHI_REFERENCE;
+  c->_plan = new VisPlan();
+  (void)hipGetDevice(&c->_plan->device);
+  if (!alloc_freq_image(c) || !alloc_samples(c)) {
+    const int code = vis_get_last_error_code();
+    const std::string msg = vis_get_last_error();
+    vis_config_destroy(c);
+    set_error(code, msg);
+    return nullptr;
+  }
+  vis_config_compute_scale(c);
+  return c;
+}
+
+VisConfig* vis_config_get_default() {
+  return vis_config_create(true, make_float2(0.1f, 0.1f), make_int2(128, 128), make_float3(1.0f, 0.0f, 0.0f),
+                           make_float3(0.0f, 1.0f, 0.0f), make_int2(512, 512), 256, 1);
+}
+
+bool vis_config_check(VisConfig* vis_config) {
+  if (!vis_config) return false;
+  std::string why;
+  return config_valid(vis_config, &why);
+}
+
+void vis_config_manual_d_image(VisConfig* vis_config, float* d_image) {
+  if (!vis_config) return;
+  vis_config->_d_image = d_image;
+}
+
+void vis_config_change_number_of_samples(VisConfig* c, int2 number_of_samples) {
+  clear_error();
+  if (!c) return;
+  if (number_of_samples.x <= 0 || number_of_samples.y <= 0) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_config_change_number_of_samples: non-positive size");
+    return;
+  }
+  (void)hipStreamSynchronize(c->_plan->stream);
+  c->_number_of_samples = number_of_samples;
+  alloc_samples(c);
+}
+
+void vis_config_change_cutoff_frequency(VisConfig* c, int2 cutoff_frequency) {
+  clear_error();
+  if (!c) return;
+  if (cutoff_frequency.x < 0 || cutoff_frequency.y < 0) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_config_change_cutoff_frequency: negative cutoff");
+    return;
+  }
+  (void)hipStreamSynchronize(c->_plan->stream);
+  c->_cutoff_frequency = cutoff_frequency;
+  alloc_freq_image(c);
+}
+
+void vis_config_change_number_of_partial_sums(VisConfig* c, int number_of_partial_sums) {
+  clear_error();
+  if (!c) return;
+  if (number_of_partial_sums < 1) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_config_change_number_of_partial_sums: value < 1");
+    return;
+  }
+  (void)hipStreamSynchronize(c->_plan->stream);
+  c->_number_of_partial_sums = number_of_partial_sums;
+  alloc_freq_image(c);
+}
+
+void vis_config_destroy(VisConfig* c) {
+  if (!c) return;
+  if (c->_plan) (void)hipStreamSynchronize(c->_plan->stream);
+  if (c->_automatic_d_image && c->_d_image) (void)hipFree(c->_d_image);
+  if (c->_d_freq_image_arranged) (void)hipFree(c->_d_freq_image_arranged);
+  if (c->_d_freq_image) (void)hipFree(c->_d_freq_image);
+  plan_destroy(c->_plan);
+  std::free(c);
+}
+
+void vis_config_compute_scale(VisConfig* c) {
+  if (!c) return;
+  c->_scale = c->step_size.x * c->step_size.y;
+}
+
+// Registration: zero-padded AoS copies, as load_into_device (meshless_vis.cu:264-290).
+void vis_register_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+  clear_error();
+  if (!c || !ds) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_register_meshless_dataset: null argument");
+    return;
+  }
+  const long long multiple = static_cast<long long>(c->_number_of_partial_sums) * c->block_length;
+  if (multiple <= 0) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "vis_register_meshless_dataset: number_of_partial_sums*block_length <= 0");
+    return;
+  }
+  hipStream_t s = c->_plan->stream;
+  for (int j = 0; j < ds->number_of_groups; ++j) {
+    Group& g = ds->groups[j];
+    g.d_constraints = nullptr;
+    g.d_radii = nullptr;
+    const int n = g.number_of_terms;
+    if (n < 0) {
+      set_error(VIS_ERROR_INVALID_VALUE, "negative number_of_terms");
+      return;
+    }
+    const long long rounded = ((n + multiple - 1) / multiple) * multiple;
+    g.d_number_of_terms = static_cast<int>(rounded);
+    GroupInfo info;
+    info.n = n;
+    info.padded = static_cast<int>(rounded);
+    if (g.h_constraints && rounded > 0) {
+      if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_constraints), sizeof(Constraint) * rounded), "hipMalloc(d_constraints)"))
+        return;
+      if (!hip_ok(hipMemsetAsync(g.d_constraints, 0, sizeof(Constraint) * rounded, s), "hipMemset") ||
+          !hip_ok(hipMemcpyAsync(g.d_constraints, g.h_constraints, sizeof(Constraint) * n, hipMemcpyHostToDevice, s),
+                  "hipMemcpy(d_constraints)"))
+        return;
+      float wmax = 0.0f;
+      for (int i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(g.h_constraints[i].weight));
+      info.w_absmax = wmax;
+    }
+    if (g.h_radii && rounded > 0) {
+      if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_radii), sizeof(float) * rounded), "hipMalloc(d_radii)")) return;
+      if (!hip_ok(hipMemsetAsync(g.d_radii, 0, sizeof(float) * rounded, s), "hipMemset") ||
+          !hip_ok(hipMemcpyAsync(g.d_radii, g.h_radii, sizeof(float) * n, hipMemcpyHostToDevice, s), "hipMemcpy(d_radii)"))
+        return;
+      info.has_radii = true;
+      float lo = n ? g.h_radii[0] : 1.0f, hi = lo;
+      for (int i = 1; i < n; ++i) {
+        lo = std::min(lo, g.h_radii[i]);
+        hi = std::max(hi, g.h_radii[i]);
+      }
+      info.h_min = lo;
+      info.h_max = hi;
+      info.uniform_radius = (lo == hi);
+      info.radius = lo;
+    } else {
+      info.has_radii = false;
+      info.uniform_radius = true;
+      info.radius = 1.0f;
+    }
+    if (!hip_ok(hipStreamSynchronize(s), "register synchronize")) return;
+    if (g.d_constraints) registry_put(g.d_constraints, info);
+  }
+}
+
+void vis_unregister_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+  clear_error();
+  if (!ds) return;
+  if (c && c->_plan) (void)hipStreamSynchronize(c->_plan->stream);
+  for (int j = 0; j < ds->number_of_groups; ++j) {
+    Group& g = ds->groups[j];
+    if (g.d_constraints) {
+      registry_erase(g.d_constraints);
+      mfma_forget_group(g.d_constraints);
+      (void)hipFree(g.d_constraints);
+    }
+    if (g.d_radii) (void)hipFree(g.d_radii);
+    g.d_constraints = nullptr;
+    g.d_radii = nullptr;
+  }
+}
+
+void fourier_transform(MeshlessDataset* ds, VisConfig* c) {
+  clear_error();
+  if (!ds || !c) {
+    set_error(VIS_ERROR_INVALID_VALUE, "fourier_transform: null argument");
+    return;
+  }
+  if (ds->number_of_groups < 1) return;  // as fourier_transform.cu:163
+  vis_config_compute_scale(c);
+  std::string why;
+  if (!config_valid(c, &why)) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+    return;
+  }
+  spectrum(ds, c);
+}
+
+void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
+  clear_error();
+  if (!ds || !c) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering: null argument");
+    return;
+  }
+  std::string why;
+  if (!config_valid(c, &why)) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+    return;
+  }
+  if (!c->_d_image) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering: no _d_image (manual image not set)");
+    return;
+  }
+  VisPlan* p = c->_plan;
+  const size_t nfreq = spectrum_count(c);
+  if (ds->number_of_groups < 1) {
+    // No groups: the reference leaves the spectrum as it was; an empty dataset renders zero here.
+    if (nfreq && !hip_ok(hipMemsetAsync(c->_d_freq_image, 0, sizeof(float2) * nfreq, p->stream), "hipMemset")) return;
+  } else {
+    vis_config_compute_scale(c);
+    if (!spectrum(ds, c)) return;
+  }
+  hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
+                    c->_number_of_samples.y, p->tw_x, p->tw_y, c->_d_freq_image_arranged, c->_d_image, p->stream),
+         "c2r launch");
+}
+
+void vis_opengl_fourier_volume_rendering(MeshlessDataset*, VisConfig*, unsigned int) {
+  set_error(VIS_ERROR_UNSUPPORTED, "vis_opengl_fourier_volume_rendering: no OpenGL interop in this build");
+}
+
+void vis_copy_to_host(VisConfig* c, float* h_image) {
+  clear_error();
+  if (!c || !h_image || !c->_d_image) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_copy_to_host: null argument or no image");
+    return;
+  }
+  const size_t n = static_cast<size_t>(c->_number_of_samples.x) * c->_number_of_samples.y;
+  if (!hip_ok(hipMemcpyAsync(h_image, c->_d_image, sizeof(float) * n, hipMemcpyDeviceToHost, c->_plan->stream),
+              "hipMemcpy(image)"))
+    return;
+  hip_ok(hipStreamSynchronize(c->_plan->stream), "synchronize");
+}
+
+void vis_copy_spectrum_to_host(VisConfig* c, float2* h_spectrum) {
+  clear_error();
+  if (!c || !h_spectrum) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_copy_spectrum_to_host: null argument");
+    return;
+  }
+  const size_t n = spectrum_count(c);
+  if (n == 0) return;
+  if (!hip_ok(hipMemcpyAsync(h_spectrum, c->_d_freq_image, sizeof(float2) * n, hipMemcpyDeviceToHost, c->_plan->stream),
+              "hipMemcpy(spectrum)"))
+    return;
+  hip_ok(hipStreamSynchronize(c->_plan->stream), "synchronize");
+}
+
+void vis_set_stream(VisConfig* c, hipStream_t stream) {
+  if (!c) return;
+  c->_plan->stream = stream;
+}
+
+hipStream_t vis_get_stream(VisConfig* c) { return c ? c->_plan->stream : nullptr; }
+
+void vis_synchronize(VisConfig* c) {
+  clear_error();
+  if (!c) return;
+  hip_ok(hipStreamSynchronize(c->_plan->stream), "synchronize");
+}
+
+void vis_fourier_volume_rendering_batched(MeshlessDataset** datasets, VisConfig** configs, int n) {
+  clear_error();
+  if (n < 0 || (n > 0 && (!datasets || !configs))) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_batched: bad arguments");
+    return;
+  }
+  for (int i = 0; i < n; ++i) {
+    vis_fourier_volume_rendering(datasets[i], configs[i]);
+    if (vis_get_last_error_code() != VIS_SUCCESS) return;
+  }
+}
+
+void vis_set_kernel_timing(VisConfig* c, int enable) {
+  if (!c) return;
+  c->_plan->timing = enable != 0;
+}
+
+int vis_get_kernel_timing(VisConfig* c, double* total_ms) {
+  clear_error();
+  if (!c) return 0;
+  VisPlan* p = c->_plan;
+  if (!hip_ok(hipStreamSynchronize(p->stream), "synchronize")) return 0;
+  double sum = 0.0;
+  for (size_t i = 0; i < p->events_used; ++i) {
+    float ms = 0.0f;
+    if (!hip_ok(hipEventElapsedTime(&ms, p->events[i].first, p->events[i].second), "hipEventElapsedTime")) return 0;
+    sum += ms;
+  }
+  if (total_ms) *total_ms = sum;
+  return static_cast<int>(p->events_used);
+}
+
+void vis_reset_kernel_timing(VisConfig* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->_plan->stream);
+  c->_plan->events_used = 0;
+}
+
+int vis_get_last_error_code(void) { return g_err_code; }
+const char* vis_get_last_error(void) { return g_err_msg.c_str(); }
+void vis_clear_last_error(void) { clear_error(); }
+
+}  // extern "C"
