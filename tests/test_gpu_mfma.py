@@ -1,0 +1,82 @@
+"""GPU parity of the MFMA complex-GEMM forms against the FP64 oracle (through the C ABI).
+
+Tolerances (relative L2 of the spectrum against the FP64 oracle of the same Phi):
+  hi/lo split FP16 (3 products)      <= 1e-5   (the accuracy target)
+  hi/lo split BF16 (3 products)      <= 3e-5   (16 significant bits per operand)
+  single FP16 product                <= 5e-3   (11 bits; reported for comparison)
+"""
+import numpy as np
+import pytest
+
+import oracle
+from libmeshlessvis_amd import host
+from libmeshlessvis_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"mfma_split_f16": 1e-5, "mfma_split_bf16": 3e-5, "mfma_f16": 5e-3}
+
+
+def gpu_spectrum(groups, cutoff, samples, step, u, v, phi_mode, form):
+    ds = host.Dataset(groups)
+    with host.Config(cutoff=cutoff, samples=samples, step=step, u=u, v=v, form=form, phi_mode=phi_mode) as cfg:
+        cfg.register(ds)
+        try:
+            cfg.render(ds)
+            return cfg.spectrum(), cfg.image()
+        finally:
+            cfg.unregister(ds)
+
+
+def cloud(rng, n, basis, radii):
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    w = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    return [(pos, w, radii, basis)]
+
+
+U = np.array([0.6, 0.8, 0.0])
+V = np.array([-0.48, 0.36, 0.8])
+
+
+@pytest.mark.parametrize("form", ["mfma_split_f16", "mfma_split_bf16", "mfma_f16"])
+@pytest.mark.parametrize("kind", ["gauss_var_analytic", "gauss_var_reference", "wendland_const", "sph_none",
+                                  "gauss_const_reference"])
+def test_forms_small(form, kind):
+    rng = np.random.default_rng(7)
+    n = 3000
+    if kind == "gauss_var_analytic":
+        basis, radii, mode = "GAUSSIAN", rng.uniform(0.005, 0.05, n).astype(np.float32), "analytic"
+    elif kind == "gauss_var_reference":
+        basis, radii, mode = "GAUSSIAN", rng.uniform(0.002, 0.01, n).astype(np.float32), "reference"
+    elif kind == "wendland_const":
+        basis, radii, mode = "WENDLAND_D3_C2", np.full(n, 0.03, np.float32), "analytic"
+    elif kind == "sph_none":
+        basis, radii, mode = "SPH", None, "reference"
+    else:
+        basis, radii, mode = "GAUSSIAN", np.full(n, 0.01, np.float32), "reference"
+    groups = cloud(rng, n, basis, radii)
+    cutoff, samples, step = (64, 48), (256, 128), (0.5, 0.5)
+    print(f"[case] {form} {kind}", flush=True)
+    F, img = gpu_spectrum(groups, cutoff, samples, step, U, V, mode, form)
+    F64 = oracle.spectrum64(groups, cutoff, step, U, V, mode)
+    e = oracle.rel_l2(F, F64)
+    print(f"{form} {kind}: rel-L2 {e:.3e}")
+    assert np.isfinite(F).all()
+    assert e <= TOL[form]
+    img64 = oracle.c2r64(oracle.arrange64(F64, *cutoff, *samples), *samples)
+    assert oracle.rel_l2(img, img64) <= TOL[form] * 1.5
+
+
+def test_split_f16_matches_plain_on_clustered_subset():
+    wl = W.lognormal_field(n=65536, grid=64)
+    vw = wl.views[0]
+    Fm, _ = gpu_spectrum([wl.group()], wl.cutoff, wl.samples, wl.step, vw.u, vw.v, wl.phi_mode, "mfma_split_f16")
+    Fp, _ = gpu_spectrum([wl.group()], wl.cutoff, wl.samples, wl.step, vw.u, vw.v, wl.phi_mode, "plain")
+    e = oracle.rel_l2(Fm, Fp)
+    print(f"lognormal 65536 split-f16 vs plain: {e:.3e}")
+    assert e <= 1e-5
+    rng = np.random.default_rng(1)
+    idx = rng.choice(Fm.size, 256, replace=False)
+    F64 = oracle.spectrum64([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, wl.phi_mode, idx=idx)
+    full = np.concatenate([Fm.reshape(-1)[idx]])
+    assert oracle.rel_l2(full, F64) <= 1e-5
