@@ -121,6 +121,35 @@ def cpu_baseline(wl, threads: int):
     }
 
 
+def kernel_name(wl, form: str) -> str:
+    if form == "plain":
+        return "direct_sum_kernel"
+    varying = wl.radii is not None and float(wl.radii.min()) != float(wl.radii.max())
+    return "mfma_binned_kernel" if varying and wl.basis != "GAUSSIAN" else "mfma_sum_kernel"
+
+
+def measured_traffic(workload: str, form: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 --pmc summary, if any."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    entry = json.loads(f.read_text()).get(f"{workload}/{form}")
+    return entry["bytes"] if entry else None
+
+
+def calibrate_copy(mib: int) -> None:
+    """One elementwise copy of a known size on the GPU, for scaling FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes_read": 4 * n, "calibrate_copy_bytes_written": 4 * n}), flush=True)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,9 +158,16 @@ def main() -> None:
     ap.add_argument("--workload", default="lognormal_field")
     ap.add_argument("--n", type=int, default=None, help="override the particle count")
     ap.add_argument("--form", default="auto")
+    ap.add_argument("--views", type=int, default=0, help="use only the first K views (0: all)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (views would shard "
               "across GPUs with no communication, see the design notes)", file=sys.stderr)
@@ -144,7 +180,9 @@ def main() -> None:
     wl = make_workload(args.workload, args.n)
     log(f"workload {wl.name}: N={wl.n} cutoff={wl.cutoff} samples={wl.samples} views={len(wl.views)} "
         f"({time.perf_counter() - t:.1f} s to generate)")
-    form = "plain" if args.form == "auto" else args.form
+    form = "mfma_split_f16" if args.form == "auto" else args.form
+    if args.views:
+        wl.views = wl.views[: args.views]
 
     ds = host.Dataset([wl.group()])
     cfgs = [host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
@@ -156,6 +194,18 @@ def main() -> None:
     terms = views * wl.terms_per_view
     value = terms / elapsed
     log(f"{form}: {elapsed:.3f} s for {views} views -> {value:.3e} terms/s; kernel {kernel_ms / max(1, launches):.3f} ms/launch")
+
+    plain = None
+    if form != "plain" and not args.no_plain:
+        pcfg = host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=wl.views[0].u, v=wl.views[0].v,
+                           form="plain", phi_mode=wl.phi_mode)
+        p_elapsed, p_launches, p_ms = time_renders([pcfg], ds, 1, 0)
+        pcfg.close()
+        p_value = wl.terms_per_view / p_elapsed
+        plain = {"value": p_value, "unit": "terms/s", "kernel_ms_per_launch": p_ms / max(1, p_launches),
+                 "views": 1, "speedup": value / p_value}
+        log(f"plain FP32: {p_value:.3e} terms/s ({p_ms / max(1, p_launches):.1f} ms/view); "
+            f"{form} is {value / p_value:.1f}x")
 
     rel = None
     if not args.no_accuracy:
@@ -183,8 +233,8 @@ def main() -> None:
         "peak": peak,
         "unit": "TFLOP/s",
         "frac": (achieved_tflops / peak) if achieved_tflops else None,
-        "traffic": None,
-        "kernel": "direct_sum_kernel" if form == "plain" else "mfma_sum_kernel",
+        "traffic": measured_traffic(wl.name, form),
+        "kernel": kernel_name(wl, form),
         "kernel_ms_per_launch": avg_ms,
     }
 
@@ -224,6 +274,7 @@ def main() -> None:
         },
         "views_per_s": views / elapsed,
         "rel_l2_vs_fp64": rel,
+        "plain_fp32": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -23,7 +23,7 @@ BUILD = ROOT / "_build" / "libmeshless_vis"
 LIB = PKG / "libmeshless_vis.so"
 ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
 
-SOURCES = ["vis_runtime.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip"]
+SOURCES = ["vis_runtime.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
 
 
 def _hipcc() -> str:

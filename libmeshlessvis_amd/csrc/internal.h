@@ -71,6 +71,10 @@ struct ViewParams {
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
                                double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s);
 
+hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
+                                      int n_slots, double sx, double sy, double3 u, double3 v, Packet* out,
+                                      hipStream_t s);
+
 // direct_sum.hip: plain FP32 direct sum over [first, first+count) packets.
 //   phi_kind: 0 = uniform radius (Phi hoisted per frequency), 1 = per-particle radius.
 //   partial: [ksplit][2cx*cy] double2, slot k holds the sum of split k (unscaled, +i sin).

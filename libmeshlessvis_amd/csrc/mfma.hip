@@ -30,80 +30,9 @@
 //  * split products: hi*hi + hi*lo + lo*hi into the same FP32 accumulators;
 //  * accumulation: MFMA FP32 accumulators are folded into a second FP32 set
 //    every 512 particles; split-K partials (FP64) are reduced in order.
-#include "mfma.h"
-#include "phi.h"
+#include "mfma_common.h"
 
 namespace mv {
-
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
-typedef float f32x16_t __attribute__((ext_vector_type(16)));
-typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
-
-constexpr int kStep = 16;          // particles per LDS stage
-constexpr int kPitch = kStep + 4;  // words per LDS row (padding against bank conflicts)
-constexpr int kTileX = 128;        // workgroup tile along x (A rows)
-constexpr int kTileY = 128;        // workgroup tile along y (B columns)
-constexpr int kThreads = 512;      // 8 waves: 2 along x (64 rows each) x 4 along y (32 columns each)
-constexpr int kFlushSteps = 32;    // fold the MFMA accumulators every 512 particles
-constexpr float kTwoPowM32 = 2.3283064365386962890625e-10f;
-constexpr float kLog2e = 1.44269504088896340736f;
-
-enum OperandType { OP_F16 = 0, OP_BF16 = 1 };
-
-struct MfmaParams {
-  int cx, cy;
-  float sx, sy;
-  float sigma_pi_log2e;  // Gaussian: +-pi*log2(e); 0 for the uniform kind
-  float hh_ref;          // h_ref^2
-  float w_inv;           // 1/wmax (0 when every weight is zero)
-  float w_max;
-  float radius;          // uniform kind: Phi(r*radius)
-  int basis, phi_mode;
-  int tiles_x;
-  int steps_total;       // 16-particle steps over the group
-  int steps_per_split;
-};
-
-template <int OP>
-__device__ __forceinline__ uint32_t pack2(float a, float b) {
-  if (OP == OP_F16) {
-    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
-  } else {
-    const __bf16 ha = static_cast<__bf16>(a), hb = static_cast<__bf16>(b);
-    return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, ha)) |
-           (static_cast<uint32_t>(__builtin_bit_cast(uint16_t, hb)) << 16);
-  }
-}
-
-template <int OP>
-__device__ __forceinline__ void unpack2(uint32_t u, float* a, float* b) {
-  if (OP == OP_F16) {
-    const auto h = __builtin_bit_cast(_Float16 __attribute__((ext_vector_type(2))), u);
-    *a = static_cast<float>(h[0]);
-    *b = static_cast<float>(h[1]);
-  } else {
-    *a = __builtin_bit_cast(float, u << 16);
-    *b = __builtin_bit_cast(float, u & 0xffff0000u);
-  }
-}
-
-template <int OP>
-__device__ __forceinline__ f32x16_t mfma(u32x4_t a, u32x4_t b, f32x16_t c) {
-  if (OP == OP_F16)
-    return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(half8_t, a), __builtin_bit_cast(half8_t, b), c,
-                                                  0, 0, 0);
-  else
-    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
-                                                   c, 0, 0, 0);
-}
-
-__device__ __forceinline__ u32x4_t negate_imag(u32x4_t v) {  // (r, i) -> (r, -i) per word
-  return v ^ (u32x4_t){0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
-}
-__device__ __forceinline__ u32x4_t swap_halves(u32x4_t v) {  // (r, i) -> (i, r) per word
-  return (v >> 16) | (v << 16);
-}
 
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
 __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __restrict__ pk, MfmaParams p,
@@ -326,10 +255,7 @@ bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const Vie
                          EventPair* ev) {
   const bool uniform = info.uniform_radius;
   const bool gauss = !uniform && vp.basis == GAUSSIAN;
-  if (!uniform && !gauss) {
-    set_error(VIS_ERROR_UNSUPPORTED, "MFMA form: per-particle radii need the binned form for SPH/Wendland");
-    return false;
-  }
+  if (!uniform && !gauss) return mfma_binned_group_spectrum(g, info, n, vp, u, v, c, first, s, ws, acc, ev);
   const int nfreq = 2 * vp.cx * vp.cy;
   const int n_padded = ((n + 255) / 256) * 256;
   if (!ws->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
@@ -374,11 +300,15 @@ bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const Vie
     launch_kind<OP_F16, true>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), gauss);
   if (!hip_ok(hipGetLastError(), "mfma_sum launch")) return false;
   if (ev && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
-  hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, ws->partial.as<double2>(), ksplit,
-                     vp.cx, vp.cy, first ? 1 : 0, acc);
+  return launch_mfma_reduce(ws->partial.as<double2>(), ksplit, vp.cx, vp.cy, first, acc, s);
+}
+
+bool launch_mfma_reduce(const double2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s) {
+  const int nfreq = 2 * cx * cy;
+  hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, partial, ksplit, cx, cy,
+                     first ? 1 : 0, acc);
   return hip_ok(hipGetLastError(), "mfma_reduce launch");
 }
 
-void mfma_forget_group(const void*) {}
 
 }  // namespace mv

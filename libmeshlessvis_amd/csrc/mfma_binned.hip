@@ -1,0 +1,399 @@
+// MFMA form for SPH / Wendland groups with per-particle radii.
+//
+// Phi(r h_i) does not factor into x and y parts for these bases, so the
+// particles are sorted into log-spaced bins of h (once per registration) and,
+// inside bin b, Phi is interpolated in h through T Chebyshev nodes h_bt:
+//
+//     Phi(r h_i) ~= sum_t l_bt(h_i) Phi(r h_bt)          (l_bt: Lagrange basis)
+//     S(x, y)    ~= sum_b sum_t Phi(r h_bt) * sum_{i in b} A(x, i) B_t(i, y)
+//     A(x, i) = exp(+2 pi i x a_i),  B_t(i, y) = w_i' l_bt(h_i) exp(+2 pi i y b_i)
+//
+// i.e. one complex GEMM per (bin, node) sharing the A operand; the node factors
+// Phi(r h_bt) are applied when the accumulators are folded (every 1024
+// particles and at every bin change).  The interpolation error per term is
+// bounded by max|Phi^(T)| (r dh)^T / (T! 2^(2T-1)); with bins of ratio 1.1 and
+// T = 4 it is below 2e-6 of Phi(0).
+//
+// Kernel: 8 waves, 128 (x) x 64 (y) complex tile per workgroup, 32 x 32 per
+// wave, T accumulator pairs per wave; operand generation and split products
+// as in mfma.hip.
+#include "mfma_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+#include <vector>
+
+namespace mv {
+
+constexpr int kBTileX = 128;
+constexpr int kBTileY = 64;
+constexpr int kBFlushSteps = 64;  // fold every 1024 particles
+constexpr int kMaxNodes = 5;
+
+struct BinnedParams {
+  int cx, cy;
+  float sx, sy;
+  float w_inv, w_max;
+  int tiles_x;
+  int steps_total;
+  int steps_per_split;
+  const uint16_t* step_bin;  // bin of every 16-particle step
+  const float* nodes;        // [bin][kMaxNodes] Chebyshev nodes in h
+  const float* inv_den;      // [bin][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+};
+
+template <int OP, bool SPLIT, int T, int BASIS, int MODE>
+__global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const Packet* __restrict__ pk, BinnedParams p,
+                                                                    double2* __restrict__ partial) {
+  __shared__ uint32_t ldsA[2][2][kBTileX * kPitch];     // [buf][hi, lo]
+  __shared__ uint32_t ldsB[2][T][2][kBTileY * kPitch];  // [buf][node][hi, lo]
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wx = wave & 3;   // 32-row block
+  const int wy = wave >> 2;  // 32-column block
+  const int j0 = (blockIdx.x % p.tiles_x) * kBTileX;
+  const int y0 = (blockIdx.x / p.tiles_x) * kBTileY;
+  const int ks = blockIdx.y;
+
+  const int gp = tid & (kStep - 1);
+  const int gg = tid >> 4;  // 0..31
+  uint32_t ux[4], uy[2];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) ux[m] = static_cast<uint32_t>(j0 + gg + 32 * m - p.cx + 1);
+#pragma unroll
+  for (int m = 0; m < 2; ++m) uy[m] = static_cast<uint32_t>(y0 + gg + 32 * m);
+
+  auto generate = [&](int step, int buf) {
+    const Packet q = pk[static_cast<size_t>(step) * kStep + gp];
+    const int b = p.step_bin[step];
+    const float* nd = p.nodes + b * kMaxNodes;
+    const float* id = p.inv_den + b * kMaxNodes;
+    float l[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      float prod = id[t];
+#pragma unroll
+      for (int s = 0; s < T; ++s)
+        if (s != t) prod *= (q.h - nd[s]);
+      l[t] = prod;
+    }
+    const float wq = q.w * p.w_inv;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const uint32_t P = ux[m] * q.pu;
+      const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+      const float re = cos_rev(ph), im = sin_rev(ph);
+      const uint32_t hi = pack2<OP>(re, im);
+      const int o = (gg + 32 * m) * kPitch + gp;
+      ldsA[buf][0][o] = hi;
+      if (SPLIT) {
+        float hr, hm;
+        unpack2<OP>(hi, &hr, &hm);
+        ldsA[buf][1][o] = pack2<OP>(re - hr, im - hm);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const uint32_t P = uy[m] * q.pv;
+      const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+      const float re0 = wq * cos_rev(ph), im0 = wq * sin_rev(ph);
+      const int o = (gg + 32 * m) * kPitch + gp;
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const float re = l[t] * re0, im = l[t] * im0;
+        const uint32_t hi = pack2<OP>(re, im);
+        ldsB[buf][t][0][o] = hi;
+        if (SPLIT) {
+          float hr, hm;
+          unpack2<OP>(hi, &hr, &hm);
+          ldsB[buf][t][1][o] = pack2<OP>(re - hr, im - hm);
+        }
+      }
+    }
+  };
+
+  f32x16_t acc[T][2], sum[2];
+#pragma unroll
+  for (int t = 0; t < T; ++t) acc[t][0] = acc[t][1] = (f32x16_t){};
+  sum[0] = sum[1] = (f32x16_t){};
+
+  auto consume = [&](int buf) {
+#pragma unroll
+    for (int kk = 0; kk < kStep / 8; ++kk) {
+      const int pofs = kk * 8 + (lane >> 5) * 4;
+      const int row = wx * 32 + (lane & 31);
+      const int col = wy * 32 + (lane & 31);
+      const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kPitch + pofs]);
+      u32x4_t alo;
+      if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kPitch + pofs]);
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][0][col * kPitch + pofs]);
+        const u32x4_t bre_hi = negate_imag(bhi), bim_hi = swap_halves(bhi);
+        acc[t][0] = mfma<OP>(ahi, bre_hi, acc[t][0]);
+        acc[t][1] = mfma<OP>(ahi, bim_hi, acc[t][1]);
+        if (SPLIT) {
+          const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][1][col * kPitch + pofs]);
+          acc[t][0] = mfma<OP>(ahi, negate_imag(blo), acc[t][0]);
+          acc[t][1] = mfma<OP>(ahi, swap_halves(blo), acc[t][1]);
+          acc[t][0] = mfma<OP>(alo, bre_hi, acc[t][0]);
+          acc[t][1] = mfma<OP>(alo, bim_hi, acc[t][1]);
+        }
+      }
+    }
+  };
+
+  const int y = y0 + wy * 32 + (lane & 31);
+  const float fv = p.sy * static_cast<float>(y);
+  // sum += sum_t Phi(r h_bt) acc_t; acc = 0.
+  float rr[16];  // |k| of the lane's 16 outputs
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+    const float fu = p.sx * static_cast<float>(j - p.cx + 1);
+    rr[i] = sqrtf(fu * fu + fv * fv);
+  }
+  auto fold = [&](int b) {
+    const float* nd = p.nodes + b * kMaxNodes;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const float hn = nd[t];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const float f = phi<BASIS, MODE>(rr[i] * hn);
+        sum[0][i] = fmaf(f, acc[t][0][i], sum[0][i]);
+        sum[1][i] = fmaf(f, acc[t][1][i], sum[1][i]);
+        __builtin_amdgcn_sched_barrier(0);  // evaluate one output at a time: keeps the fold's registers few
+      }
+      acc[t][0] = acc[t][1] = (f32x16_t){};
+    }
+  };
+
+  const int s_begin = ks * p.steps_per_split;
+  const int s_end = min(p.steps_total, s_begin + p.steps_per_split);
+  int cur = (s_begin < s_end) ? p.step_bin[s_begin] : 0;
+  int since = 0;
+  if (s_begin < s_end) generate(s_begin, 0);
+  __syncthreads();
+  for (int s = s_begin; s < s_end; ++s) {
+    const int buf = (s - s_begin) & 1;
+    if (s + 1 < s_end) generate(s + 1, buf ^ 1);
+    const int b = p.step_bin[s];
+    if (b != cur) {
+      fold(cur);
+      cur = b;
+      since = 0;
+    }
+    consume(buf);
+    if (++since == kBFlushSteps) {
+      fold(cur);
+      since = 0;
+    }
+    __syncthreads();
+  }
+  if (since) fold(cur);
+
+  if (y >= p.cy) return;
+  const int two_cx = 2 * p.cx;
+  const double wm = static_cast<double>(p.w_max);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+    if (j >= two_cx) continue;
+    partial[(static_cast<size_t>(ks) * two_cx + j) * p.cy + y] =
+        make_double2(static_cast<double>(sum[0][i]) * wm, static_cast<double>(sum[1][i]) * wm);
+  }
+}
+
+// ------------------------------------------------------------------ host ---
+
+// Per-group binning, built once per registration and kept until unregister.
+struct BinCache {
+  int n = 0, nb = 0, T = 0;
+  int n_slots = 0;
+  DevBuf slot_map, step_bin, nodes, inv_den;
+  void release() {
+    slot_map.release();
+    step_bin.release();
+    nodes.release();
+    inv_den.release();
+  }
+};
+
+static std::mutex g_bin_mu;
+static std::unordered_map<const void*, BinCache>& bin_caches() {
+  static std::unordered_map<const void*, BinCache> m;
+  return m;
+}
+
+void mfma_forget_group(const void* key) {
+  std::lock_guard<std::mutex> lk(g_bin_mu);
+  auto it = bin_caches().find(key);
+  if (it != bin_caches().end()) {
+    it->second.release();
+    bin_caches().erase(it);
+  }
+}
+
+static int auto_bins(float h_min, float h_max) {
+  if (!(h_min > 0.0f) || !(h_max > h_min)) return 1;
+  const double nb = std::ceil(std::log(static_cast<double>(h_max) / h_min) / std::log(1.1));
+  return static_cast<int>(std::min(1024.0, std::max(1.0, nb)));
+}
+
+// Sort the group's particles into nb bins of h (log-spaced when h_min > 0),
+// each padded to whole 16-particle steps, with T Chebyshev nodes per bin.
+static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s, BinCache* bc) {
+  std::vector<float> h(n);
+  if (n > 0) {
+    if (!hip_ok(hipMemcpyAsync(h.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s), "hipMemcpy(radii)"))
+      return false;
+    if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
+  }
+  const double lo = info.h_min, hi = info.h_max;
+  const bool logb = lo > 0.0;
+  std::vector<double> edge(nb + 1);
+  for (int b = 0; b <= nb; ++b)
+    edge[b] = logb ? lo * std::pow(hi / lo, static_cast<double>(b) / nb) : lo + (hi - lo) * b / nb;
+  auto bin_of = [&](float x) {
+    double f = logb ? std::log(x / lo) / std::log(hi / lo) : (x - lo) / (hi - lo);
+    int b = static_cast<int>(f * nb);
+    return std::min(nb - 1, std::max(0, b));
+  };
+  std::vector<int> count(nb, 0);
+  for (int i = 0; i < n; ++i) ++count[bin_of(h[i])];
+  std::vector<int> start(nb + 1, 0);  // in slots, each bin padded to kStep
+  for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kStep - 1) / kStep) * kStep;
+  const int n_slots = std::max(start[nb], kStep);
+  std::vector<int> map(n_slots, -1);
+  std::vector<int> fill(start.begin(), start.end() - 1);
+  for (int i = 0; i < n; ++i) map[fill[bin_of(h[i])]++] = i;
+  const int steps = n_slots / kStep;
+  std::vector<uint16_t> sbin(steps, 0);
+  for (int b = 0; b < nb; ++b)
+    for (int k = start[b] / kStep; k < start[b + 1] / kStep; ++k) sbin[k] = static_cast<uint16_t>(b);
+  std::vector<float> nodes(static_cast<size_t>(nb) * kMaxNodes, 0.0f), inv(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
+  for (int b = 0; b < nb; ++b) {
+    const double c = 0.5 * (edge[b] + edge[b + 1]), d = 0.5 * (edge[b + 1] - edge[b]);
+    double nd[kMaxNodes];
+    for (int t = 0; t < T; ++t) nd[t] = c + d * std::cos((2.0 * t + 1.0) * M_PI / (2.0 * T));
+    for (int t = 0; t < T; ++t) {
+      double den = 1.0;
+      for (int q = 0; q < T; ++q)
+        if (q != t) den *= (nd[t] - nd[q]);
+      nodes[b * kMaxNodes + t] = static_cast<float>(nd[t]);
+      inv[b * kMaxNodes + t] = static_cast<float>(1.0 / den);
+    }
+  }
+  bc->n = n;
+  bc->nb = nb;
+  bc->T = T;
+  bc->n_slots = n_slots;
+  if (!bc->slot_map.reserve(sizeof(int) * n_slots) || !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
+      !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->inv_den.reserve(sizeof(float) * inv.size()))
+    return false;
+  return hip_ok(hipMemcpy(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots, hipMemcpyHostToDevice), "upload") &&
+         hip_ok(hipMemcpy(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps, hipMemcpyHostToDevice), "upload") &&
+         hip_ok(hipMemcpy(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size(), hipMemcpyHostToDevice), "upload") &&
+         hip_ok(hipMemcpy(bc->inv_den.ptr, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice), "upload");
+}
+
+template <int OP, bool SPLIT, int T, int BASIS, int MODE>
+static void launch_b(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial) {
+  hipLaunchKernelGGL((mfma_binned_kernel<OP, SPLIT, T, BASIS, MODE>), grid, dim3(kThreads), 0, s, pk, p, partial);
+}
+
+template <int OP, bool SPLIT, int T>
+static void launch_bt(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial, int basis,
+                      int mode) {
+  const bool an = mode == VIS_PHI_ANALYTIC;
+  if (basis == SPH)
+    an ? launch_b<OP, SPLIT, T, SPH, VIS_PHI_ANALYTIC>(grid, s, pk, p, partial)
+       : launch_b<OP, SPLIT, T, SPH, VIS_PHI_REFERENCE>(grid, s, pk, p, partial);
+  else
+    an ? launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, pk, p, partial)
+       : launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, pk, p, partial);
+}
+
+template <int OP, bool SPLIT>
+static void launch_bf(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial, int T,
+                      int basis, int mode) {
+  switch (T) {
+    case 2: launch_bt<OP, SPLIT, 2>(grid, s, pk, p, partial, basis, mode); break;
+    case 3: launch_bt<OP, SPLIT, 3>(grid, s, pk, p, partial, basis, mode); break;
+    case 4: launch_bt<OP, SPLIT, 4>(grid, s, pk, p, partial, basis, mode); break;
+    default: launch_bt<OP, SPLIT, 5>(grid, s, pk, p, partial, basis, mode); break;
+  }
+}
+
+bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u,
+                                double3 v, const VisConfig* c, bool first, hipStream_t s, MfmaWorkspace* ws,
+                                double2* acc, EventPair* ev) {
+  int T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 4;
+  if (T < 2 || T > kMaxNodes) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
+    return false;
+  }
+  int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max);
+  if (nb > 65535) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
+    return false;
+  }
+  BinCache* bc = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_bin_mu);
+    BinCache& e = bin_caches()[g.d_constraints];
+    if (e.n != n || e.nb != nb || e.T != T || !e.slot_map.ptr) {
+      e.release();
+      if (!build_bins(g, info, n, nb, T, s, &e)) {
+        e.release();
+        bin_caches().erase(g.d_constraints);
+        return false;
+      }
+    }
+    bc = &e;
+  }
+  if (!ws->packets.reserve(sizeof(Packet) * bc->n_slots)) return false;
+  if (!hip_ok(launch_prep_packets_mapped(g.d_constraints, g.d_radii, bc->slot_map.as<int>(), bc->n_slots,
+                                         c->step_size.x, c->step_size.y, u, v, ws->packets.as<Packet>(), s),
+              "prep_packets_mapped launch"))
+    return false;
+
+  BinnedParams p;
+  p.cx = vp.cx;
+  p.cy = vp.cy;
+  p.sx = vp.sx;
+  p.sy = vp.sy;
+  p.w_max = info.w_absmax;
+  p.w_inv = info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f;
+  p.tiles_x = (2 * vp.cx + kBTileX - 1) / kBTileX;
+  const int tiles_y = (vp.cy + kBTileY - 1) / kBTileY;
+  p.steps_total = bc->n_slots / kStep;
+  p.step_bin = bc->step_bin.as<uint16_t>();
+  p.nodes = bc->nodes.as<float>();
+  p.inv_den = bc->inv_den.as<float>();
+  const int tiles = p.tiles_x * tiles_y;
+  int ksplit = std::max(1, (1024 + tiles - 1) / tiles);
+  ksplit = std::min(ksplit, std::max(1, p.steps_total / 32));
+  p.steps_per_split = (p.steps_total + ksplit - 1) / ksplit;
+  ksplit = std::max(1, (p.steps_total + p.steps_per_split - 1) / p.steps_per_split);
+  const int nfreq = 2 * vp.cx * vp.cy;
+  if (!ws->partial.reserve(sizeof(double2) * static_cast<size_t>(ksplit) * nfreq)) return false;
+
+  const dim3 grid(tiles, ksplit);
+  if (ev && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
+  if (c->form == VIS_FORM_MFMA_F16)
+    launch_bf<OP_F16, false>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
+  else if (c->form == VIS_FORM_MFMA_SPLIT_BF16)
+    launch_bf<OP_BF16, true>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
+  else
+    launch_bf<OP_F16, true>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
+  if (!hip_ok(hipGetLastError(), "mfma_binned launch")) return false;
+  if (ev && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
+  return launch_mfma_reduce(ws->partial.as<double2>(), ksplit, vp.cx, vp.cy, first, acc, s);
+}
+
+}  // namespace mv

@@ -44,6 +44,43 @@ __global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __r
   out[i] = p;
 }
 
+// Same packets in a given slot order: slot k takes particle slot_map[k] (zero packet when negative).
+__global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constraint* __restrict__ c,
+                                                                  const float* __restrict__ radii,
+                                                                  const int* __restrict__ slot_map, int n_slots,
+                                                                  double sx, double sy, double3 u, double3 v,
+                                                                  Packet* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_slots) return;
+  const int i = slot_map[k];
+  Packet p;
+  if (i >= 0) {
+    const float4 q = reinterpret_cast<const float4*>(c)[i];
+    const double px = q.x, py = q.y, pz = q.z;
+    const double du = fma(u.x, px, fma(u.y, py, u.z * pz));
+    const double dv = fma(v.x, px, fma(v.y, py, v.z * pz));
+    p.pu = to_fixed_rev(sx * du);
+    p.pv = to_fixed_rev(sy * dv);
+    p.w = q.w;
+    p.h = radii ? radii[i] : 1.0f;
+  } else {
+    p.pu = 0u;
+    p.pv = 0u;
+    p.w = 0.0f;
+    p.h = 0.0f;
+  }
+  out[k] = p;
+}
+
+hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
+                                      int n_slots, double sx, double sy, double3 u, double3 v, Packet* out,
+                                      hipStream_t s) {
+  if (n_slots <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_packets_mapped_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, s, d_constraints, d_radii,
+                     slot_map, n_slots, sx, sy, u, v, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
                                double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s) {
   if (n_padded <= 0) return hipSuccess;

@@ -80,3 +80,40 @@ def test_split_f16_matches_plain_on_clustered_subset():
     F64 = oracle.spectrum64([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, wl.phi_mode, idx=idx)
     full = np.concatenate([Fm.reshape(-1)[idx]])
     assert oracle.rel_l2(full, F64) <= 1e-5
+
+
+@pytest.mark.parametrize("form", ["mfma_split_f16", "mfma_split_bf16"])
+@pytest.mark.parametrize("basis", ["WENDLAND_D3_C2", "SPH"])
+@pytest.mark.parametrize("nodes", [3, 4])
+def test_binned_variable_radii(form, basis, nodes):
+    rng = np.random.default_rng(9)
+    n = 5000
+    radii = np.exp(rng.uniform(np.log(0.004), np.log(0.15), n)).astype(np.float32)
+    groups = cloud(rng, n, basis, radii)
+    cutoff, samples, step = (64, 48), (256, 128), (0.5, 0.5)
+    ds = host.Dataset(groups)
+    with host.Config(cutoff=cutoff, samples=samples, step=step, u=U, v=V, form=form, phi_mode="analytic",
+                     nodes=nodes, h_bins=0 if nodes == 4 else 96) as cfg:
+        cfg.register(ds)
+        cfg.render(ds)
+        F = cfg.spectrum()
+        cfg.unregister(ds)
+    F64 = oracle.spectrum64(groups, cutoff, step, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"binned {form} {basis} T={nodes}: rel-L2 {e:.3e}")
+    assert e <= TOL[form]
+
+
+def test_binned_multigroup_and_reference_mode():
+    rng = np.random.default_rng(10)
+    g1 = cloud(rng, 2000, "WENDLAND_D3_C2", np.exp(rng.uniform(np.log(0.01), np.log(0.1), 2000)).astype(np.float32))[0]
+    g2 = cloud(rng, 1500, "GAUSSIAN", np.exp(rng.uniform(np.log(0.01), np.log(0.1), 1500)).astype(np.float32))[0]
+    g3 = cloud(rng, 700, "SPH", np.full(700, 0.05, np.float32))[0]
+    groups = [g1, g2, g3]
+    cutoff, samples, step = (40, 24), (96, 64), (0.7, 0.6)
+    for mode, tol in (("analytic", 1e-5), ("reference", 2e-4)):
+        F, _ = gpu_spectrum(groups, cutoff, samples, step, U, V, mode, "mfma_split_f16")
+        F64 = oracle.spectrum64(groups, cutoff, step, U, V, mode)
+        e = oracle.rel_l2(F, F64)
+        print(f"multigroup {mode}: rel-L2 {e:.3e}")
+        assert e <= tol
