@@ -68,8 +68,9 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __r
 
   auto generate = [&](int step, int buf) {
     const Packet q = pk[static_cast<size_t>(step) * kStep + gp];
-    const float dh = q.h * q.h - p.hh_ref;
-    const float wq = q.w * p.w_inv;
+    const float4 bi = p.step_bin ? p.bininfo[p.step_bin[step]] : p.bin0;
+    const float dh = q.h * q.h - (p.ref_hi ? bi.y : bi.x);
+    const float wq = q.w * bi.w;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       // A(x, i)
@@ -146,52 +147,69 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __r
     }
   };
 
+  // sum += E_b(x, y) * acc; acc = 0.  E_b = exp(s pi h_ref^2 r^2) * W_b (Gaussian), Phi(r h) * W_b (uniform).
+  const int y_out = y0 + wy * 32 + (lane & 31);
+  const float fv = p.sy * static_cast<float>(y_out);
+  auto fold = [&](int b) {
+    const float4 bi = p.step_bin ? p.bininfo[b] : p.bin0;
+    const float hh_ref = p.ref_hi ? bi.y : bi.x;
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int j = j0 + wx * 64 + rb * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const float fu = p.sx * static_cast<float>(j - p.cx + 1);
+        const float r2 = fu * fu + fv * fv;
+        float e;
+        if (GAUSS)
+          e = __builtin_amdgcn_exp2f(p.sigma_pi_log2e * hh_ref * r2) * bi.z;
+        else
+          e = phi<BASIS, MODE>(sqrtf(r2) * p.radius) * bi.z;
+        sum[rb][0][i] = fmaf(e, acc[rb][0][i], sum[rb][0][i]);
+        sum[rb][1][i] = fmaf(e, acc[rb][1][i], sum[rb][1][i]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      acc[rb][0] = acc[rb][1] = (f32x16_t){};
+    }
+  };
+
   const int s_begin = ks * p.steps_per_split;
   const int s_end = min(p.steps_total, s_begin + p.steps_per_split);
+  int cur = (p.step_bin && s_begin < s_end) ? p.step_bin[s_begin] : 0;
+  int since = 0;
   if (s_begin < s_end) generate(s_begin, 0);
   __syncthreads();
   for (int s = s_begin; s < s_end; ++s) {
     const int buf = (s - s_begin) & 1;
     if (s + 1 < s_end) generate(s + 1, buf ^ 1);
+    if (p.step_bin) {
+      const int b = p.step_bin[s];
+      if (b != cur) {
+        fold(cur);
+        cur = b;
+        since = 0;
+      }
+    }
     consume(buf);
-    if (((s - s_begin + 1) % kFlushSteps) == 0) {
-#pragma unroll
-      for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-          sum[rb][c] += acc[rb][c];
-          acc[rb][c] = (f32x16_t){};
-        }
+    if (++since == kFlushSteps) {
+      fold(cur);
+      since = 0;
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int c = 0; c < 2; ++c) sum[rb][c] += acc[rb][c];
+  if (since) fold(cur);
 
-  // Epilogue: S(x, y) = E(x, y) * sum, written as partial[ks][j][y].
+  // partial[ks][j][y]
   const int two_cx = 2 * p.cx;
-  const int y = y0 + wy * 32 + (lane & 31);
-  if (y >= p.cy) return;
-  const float fv = p.sy * static_cast<float>(y);
+  if (y_out >= p.cy) return;
 #pragma unroll
   for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int j = j0 + wx * 64 + rb * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
       if (j >= two_cx) continue;
-      const int x = j - p.cx + 1;
-      const float fu = p.sx * static_cast<float>(x);
-      float e;
-      if (GAUSS) {
-        e = __builtin_amdgcn_exp2f(p.sigma_pi_log2e * p.hh_ref * (fu * fu + fv * fv));
-      } else {
-        e = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * p.radius);
-      }
-      const double ed = static_cast<double>(e) * static_cast<double>(p.w_max);
-      partial[(static_cast<size_t>(ks) * two_cx + j) * p.cy + y] =
-          make_double2(static_cast<double>(sum[rb][0][i]) * ed, static_cast<double>(sum[rb][1][i]) * ed);
+      partial[(static_cast<size_t>(ks) * two_cx + j) * p.cy + y_out] =
+          make_double2(static_cast<double>(sum[rb][0][i]), static_cast<double>(sum[rb][1][i]));
     }
   }
 }
@@ -257,12 +275,6 @@ bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const Vie
   const bool gauss = !uniform && vp.basis == GAUSSIAN;
   if (!uniform && !gauss) return mfma_binned_group_spectrum(g, info, n, vp, u, v, c, first, s, ws, acc, ev);
   const int nfreq = 2 * vp.cx * vp.cy;
-  const int n_padded = ((n + 255) / 256) * 256;
-  if (!ws->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
-  if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
-                                  ws->packets.as<Packet>(), s),
-              "prep_packets launch"))
-    return false;
 
   MfmaParams p;
   p.cx = vp.cx;
@@ -273,14 +285,40 @@ bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const Vie
   p.phi_mode = vp.phi_mode;
   const float sigma = (vp.phi_mode == VIS_PHI_REFERENCE) ? 1.0f : -1.0f;
   p.sigma_pi_log2e = gauss ? sigma * kPi * kLog2e : 0.0f;
-  const float href = (sigma < 0.0f) ? info.h_min : info.h_max;
-  p.hh_ref = gauss ? href * href : 0.0f;
-  p.w_max = info.w_absmax;
-  p.w_inv = info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f;
+  p.ref_hi = sigma > 0.0f ? 1 : 0;
   p.radius = info.has_radii ? info.radius : 1.0f;
   p.tiles_x = (2 * vp.cx + kTileX - 1) / kTileX;
   const int tiles_y = (vp.cy + kTileY - 1) / kTileY;
-  p.steps_total = n_padded / kStep;
+  if (gauss) {
+    // Bins of h ratio 1.25: per-bin weight scale and reference radius.
+    const int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.25);
+    if (nb > 65535) {
+      set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
+      return false;
+    }
+    const BinCache* bc = bin_cache(g, info, n, nb, 1, s);
+    if (!bc) return false;
+    if (!ws->packets.reserve(sizeof(Packet) * bc->n_slots)) return false;
+    if (!hip_ok(launch_prep_packets_mapped(g.d_constraints, g.d_radii, bc->slot_map.as<int>(), bc->n_slots,
+                                           c->step_size.x, c->step_size.y, u, v, ws->packets.as<Packet>(), s),
+                "prep_packets_mapped launch"))
+      return false;
+    p.steps_total = bc->n_slots / kStep;
+    p.step_bin = bc->step_bin.as<uint16_t>();
+    p.bininfo = bc->bininfo.as<float4>();
+    p.bin0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  } else {
+    const int n_padded = ((n + 255) / 256) * 256;
+    if (!ws->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
+    if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
+                                    ws->packets.as<Packet>(), s),
+                "prep_packets launch"))
+      return false;
+    p.steps_total = n_padded / kStep;
+    p.step_bin = nullptr;
+    p.bininfo = nullptr;
+    p.bin0 = make_float4(0.0f, 0.0f, info.w_absmax, info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f);
+  }
   const int tiles = p.tiles_x * tiles_y;
   // Enough workgroups to fill 256 CUs a few times, at least 64 steps (1024 particles) per split.
   int ksplit = std::max(1, (1024 + tiles - 1) / tiles);

@@ -6,8 +6,11 @@
 //
 //     Phi(r h_i) ~= sum_t l_bt(h_i) Phi(r h_bt)          (l_bt: Lagrange basis)
 //     S(x, y)    ~= sum_b sum_t Phi(r h_bt) * sum_{i in b} A(x, i) B_t(i, y)
-//     A(x, i) = exp(+2 pi i x a_i),  B_t(i, y) = w_i' l_bt(h_i) exp(+2 pi i y b_i)
+//     A(x, i) = exp(+2 pi i x a_i),  B_t(i, y) = (w_i / W_b) l_bt(h_i) exp(+2 pi i y b_i)
 //
+// with W_b the largest |w| of bin b, folded back in with the node factors
+// (weights correlate with h, so per-bin scaling keeps the FP16 operands away
+// from the subnormal range);
 // i.e. one complex GEMM per (bin, node) sharing the A operand; the node factors
 // Phi(r h_bt) are applied when the accumulators are folded (every 1024
 // particles and at every bin change).  The interpolation error per term is
@@ -21,7 +24,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <unordered_map>
+#include <map>
+#include <tuple>
 #include <vector>
 
 namespace mv {
@@ -29,18 +33,17 @@ namespace mv {
 constexpr int kBTileX = 128;
 constexpr int kBTileY = 64;
 constexpr int kBFlushSteps = 64;  // fold every 1024 particles
-constexpr int kMaxNodes = 5;
 
 struct BinnedParams {
   int cx, cy;
   float sx, sy;
-  float w_inv, w_max;
   int tiles_x;
   int steps_total;
   int steps_per_split;
   const uint16_t* step_bin;  // bin of every 16-particle step
   const float* nodes;        // [bin][kMaxNodes] Chebyshev nodes in h
   const float* inv_den;      // [bin][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
 };
 
 template <int OP, bool SPLIT, int T, int BASIS, int MODE>
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const Packet* 
         if (s != t) prod *= (q.h - nd[s]);
       l[t] = prod;
     }
-    const float wq = q.w * p.w_inv;
+    const float wq = q.w * p.bininfo[b].w;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       const uint32_t P = ux[m] * q.pu;
@@ -158,12 +161,13 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const Packet* 
   }
   auto fold = [&](int b) {
     const float* nd = p.nodes + b * kMaxNodes;
+    const float ws = p.bininfo[b].z;
 #pragma unroll
     for (int t = 0; t < T; ++t) {
       const float hn = nd[t];
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const float f = phi<BASIS, MODE>(rr[i] * hn);
+        const float f = phi<BASIS, MODE>(rr[i] * hn) * ws;
         sum[0][i] = fmaf(f, acc[t][0][i], sum[0][i]);
         sum[1][i] = fmaf(f, acc[t][1][i], sum[1][i]);
         __builtin_amdgcn_sched_barrier(0);  // evaluate one output at a time: keeps the fold's registers few
@@ -198,73 +202,79 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const Packet* 
 
   if (y >= p.cy) return;
   const int two_cx = 2 * p.cx;
-  const double wm = static_cast<double>(p.w_max);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
     if (j >= two_cx) continue;
     partial[(static_cast<size_t>(ks) * two_cx + j) * p.cy + y] =
-        make_double2(static_cast<double>(sum[0][i]) * wm, static_cast<double>(sum[1][i]) * wm);
+        make_double2(static_cast<double>(sum[0][i]), static_cast<double>(sum[1][i]));
   }
 }
 
 // ------------------------------------------------------------------ host ---
 
-// Per-group binning, built once per registration and kept until unregister.
-struct BinCache {
-  int n = 0, nb = 0, T = 0;
-  int n_slots = 0;
-  DevBuf slot_map, step_bin, nodes, inv_den;
-  void release() {
-    slot_map.release();
-    step_bin.release();
-    nodes.release();
-    inv_den.release();
-  }
-};
-
+// Per-group binning, built once per registration and kept until unregister
+// (one entry per (bins, nodes) setting in use).
 static std::mutex g_bin_mu;
-static std::unordered_map<const void*, BinCache>& bin_caches() {
-  static std::unordered_map<const void*, BinCache> m;
+static std::map<std::tuple<const void*, int, int>, BinCache>& bin_caches() {
+  static std::map<std::tuple<const void*, int, int>, BinCache> m;
   return m;
 }
 
 void mfma_forget_group(const void* key) {
   std::lock_guard<std::mutex> lk(g_bin_mu);
-  auto it = bin_caches().find(key);
-  if (it != bin_caches().end()) {
-    it->second.release();
-    bin_caches().erase(it);
+  auto& m = bin_caches();
+  for (auto it = m.begin(); it != m.end();) {
+    if (std::get<0>(it->first) == key) {
+      it->second.release();
+      it = m.erase(it);
+    } else {
+      ++it;
+    }
   }
 }
 
-static int auto_bins(float h_min, float h_max) {
+int auto_bins(float h_min, float h_max, double ratio) {
   if (!(h_min > 0.0f) || !(h_max > h_min)) return 1;
-  const double nb = std::ceil(std::log(static_cast<double>(h_max) / h_min) / std::log(1.1));
+  const double nb = std::ceil(std::log(static_cast<double>(h_max) / h_min) / std::log(ratio));
   return static_cast<int>(std::min(1024.0, std::max(1.0, nb)));
 }
 
 // Sort the group's particles into nb bins of h (log-spaced when h_min > 0),
-// each padded to whole 16-particle steps, with T Chebyshev nodes per bin.
+// each padded to whole 16-particle steps, with T Chebyshev nodes per bin and
+// the bin's largest |w|.
 static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s, BinCache* bc) {
   std::vector<float> h(n);
+  std::vector<Constraint> cons(n);
   if (n > 0) {
-    if (!hip_ok(hipMemcpyAsync(h.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s), "hipMemcpy(radii)"))
+    if (!hip_ok(hipMemcpyAsync(h.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s), "hipMemcpy(radii)") ||
+        !hip_ok(hipMemcpyAsync(cons.data(), g.d_constraints, sizeof(Constraint) * n, hipMemcpyDeviceToHost, s),
+                "hipMemcpy(constraints)"))
       return false;
     if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
   }
   const double lo = info.h_min, hi = info.h_max;
-  const bool logb = lo > 0.0;
+  const bool logb = lo > 0.0 && hi > lo;
   std::vector<double> edge(nb + 1);
   for (int b = 0; b <= nb; ++b)
     edge[b] = logb ? lo * std::pow(hi / lo, static_cast<double>(b) / nb) : lo + (hi - lo) * b / nb;
   auto bin_of = [&](float x) {
-    double f = logb ? std::log(x / lo) / std::log(hi / lo) : (x - lo) / (hi - lo);
-    int b = static_cast<int>(f * nb);
+    if (!(hi > lo)) return 0;
+    const double f = logb ? std::log(x / lo) / std::log(hi / lo) : (x - lo) / (hi - lo);
+    const int b = static_cast<int>(f * nb);
     return std::min(nb - 1, std::max(0, b));
   };
   std::vector<int> count(nb, 0);
-  for (int i = 0; i < n; ++i) ++count[bin_of(h[i])];
+  std::vector<float4> binfo(nb);
+  for (int b = 0; b < nb; ++b)
+    binfo[b] = make_float4(static_cast<float>(edge[b] * edge[b]), static_cast<float>(edge[b + 1] * edge[b + 1]), 0.0f,
+                           0.0f);
+  for (int i = 0; i < n; ++i) {
+    const int b = bin_of(h[i]);
+    ++count[b];
+    binfo[b].z = std::max(binfo[b].z, std::fabs(cons[i].weight));
+  }
+  for (int b = 0; b < nb; ++b) binfo[b].w = binfo[b].z > 0.0f ? 1.0f / binfo[b].z : 0.0f;
   std::vector<int> start(nb + 1, 0);  // in slots, each bin padded to kStep
   for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kStep - 1) / kStep) * kStep;
   const int n_slots = std::max(start[nb], kStep);
@@ -293,12 +303,29 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   bc->T = T;
   bc->n_slots = n_slots;
   if (!bc->slot_map.reserve(sizeof(int) * n_slots) || !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
-      !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->inv_den.reserve(sizeof(float) * inv.size()))
+      !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->inv_den.reserve(sizeof(float) * inv.size()) ||
+      !bc->bininfo.reserve(sizeof(float4) * nb))
     return false;
   return hip_ok(hipMemcpy(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots, hipMemcpyHostToDevice), "upload") &&
          hip_ok(hipMemcpy(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps, hipMemcpyHostToDevice), "upload") &&
          hip_ok(hipMemcpy(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size(), hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->inv_den.ptr, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice), "upload");
+         hip_ok(hipMemcpy(bc->inv_den.ptr, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice), "upload") &&
+         hip_ok(hipMemcpy(bc->bininfo.ptr, binfo.data(), sizeof(float4) * nb, hipMemcpyHostToDevice), "upload");
+}
+
+const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s) {
+  std::lock_guard<std::mutex> lk(g_bin_mu);
+  auto key = std::make_tuple(static_cast<const void*>(g.d_constraints), nb, T);
+  BinCache& e = bin_caches()[key];
+  if (e.n != n || !e.slot_map.ptr) {
+    e.release();
+    if (!build_bins(g, info, n, nb, T, s, &e)) {
+      e.release();
+      bin_caches().erase(key);
+      return nullptr;
+    }
+  }
+  return &e;
 }
 
 template <int OP, bool SPLIT, int T, int BASIS, int MODE>
@@ -337,25 +364,13 @@ bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, co
     set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
     return false;
   }
-  int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max);
+  int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.1);
   if (nb > 65535) {
     set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
     return false;
   }
-  BinCache* bc = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_bin_mu);
-    BinCache& e = bin_caches()[g.d_constraints];
-    if (e.n != n || e.nb != nb || e.T != T || !e.slot_map.ptr) {
-      e.release();
-      if (!build_bins(g, info, n, nb, T, s, &e)) {
-        e.release();
-        bin_caches().erase(g.d_constraints);
-        return false;
-      }
-    }
-    bc = &e;
-  }
+  const BinCache* bc = bin_cache(g, info, n, nb, T, s);
+  if (!bc) return false;
   if (!ws->packets.reserve(sizeof(Packet) * bc->n_slots)) return false;
   if (!hip_ok(launch_prep_packets_mapped(g.d_constraints, g.d_radii, bc->slot_map.as<int>(), bc->n_slots,
                                          c->step_size.x, c->step_size.y, u, v, ws->packets.as<Packet>(), s),
@@ -367,14 +382,13 @@ bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, co
   p.cy = vp.cy;
   p.sx = vp.sx;
   p.sy = vp.sy;
-  p.w_max = info.w_absmax;
-  p.w_inv = info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f;
   p.tiles_x = (2 * vp.cx + kBTileX - 1) / kBTileX;
   const int tiles_y = (vp.cy + kBTileY - 1) / kBTileY;
   p.steps_total = bc->n_slots / kStep;
   p.step_bin = bc->step_bin.as<uint16_t>();
   p.nodes = bc->nodes.as<float>();
   p.inv_den = bc->inv_den.as<float>();
+  p.bininfo = bc->bininfo.as<float4>();
   const int tiles = p.tiles_x * tiles_y;
   int ksplit = std::max(1, (1024 + tiles - 1) / tiles);
   ksplit = std::min(ksplit, std::max(1, p.steps_total / 32));

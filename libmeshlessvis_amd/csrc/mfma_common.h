@@ -24,15 +24,16 @@ enum OperandType { OP_F16 = 0, OP_BF16 = 1 };
 struct MfmaParams {
   int cx, cy;
   float sx, sy;
-  float sigma_pi_log2e;  // Gaussian: +-pi*log2(e); 0 for the uniform kind
-  float hh_ref;          // h_ref^2
-  float w_inv;           // 1/wmax (0 when every weight is zero)
-  float w_max;
-  float radius;          // uniform kind: Phi(r*radius)
+  float sigma_pi_log2e;      // Gaussian: +-pi*log2(e); 0 for the uniform kind
+  float radius;              // uniform kind: Phi(r*radius)
   int basis, phi_mode;
+  int ref_hi;                // Gaussian h_ref: 0 the bin's lower edge, 1 its upper edge
   int tiles_x;
-  int steps_total;       // 16-particle steps over the group
+  int steps_total;           // 16-particle steps over the group
   int steps_per_split;
+  const uint16_t* step_bin;  // bin of every step (nullptr: one bin)
+  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
+  float4 bin0;               // the single bin when step_bin is nullptr
 };
 
 template <int OP>
@@ -74,6 +75,31 @@ __device__ __forceinline__ u32x4_t negate_imag(u32x4_t v) {  // (r, i) -> (r, -i
 __device__ __forceinline__ u32x4_t swap_halves(u32x4_t v) {  // (r, i) -> (i, r) per word
   return (v >> 16) | (v << 16);
 }
+
+// h-binning of a group (mfma_binned.hip): slot k of the per-view packet array
+// holds particle slot_map[k] (-1: zero padding); every bin starts on a
+// 16-particle step; step_bin gives each step's bin.
+constexpr int kMaxNodes = 5;
+struct BinCache {
+  int n = 0, nb = 0, T = 0;
+  int n_slots = 0;
+  DevBuf slot_map;  // int[n_slots]
+  DevBuf step_bin;  // uint16_t[n_slots / 16]
+  DevBuf nodes;     // float[nb][kMaxNodes] Chebyshev nodes in h
+  DevBuf inv_den;   // float[nb][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+  DevBuf bininfo;   // float4[nb] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
+  void release() {
+    slot_map.release();
+    step_bin.release();
+    nodes.release();
+    inv_den.release();
+    bininfo.release();
+  }
+};
+// The group's binning for (nb, T), built on first use; nullptr (error recorded) on failure.
+const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s);
+// Bins of ratio `ratio` over [h_min, h_max], at least 1, at most 1024.
+int auto_bins(float h_min, float h_max, double ratio);
 
 // acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (the MFMA kernels' partial layout).
 bool launch_mfma_reduce(const double2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s);

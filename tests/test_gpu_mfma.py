@@ -110,10 +110,27 @@ def test_binned_multigroup_and_reference_mode():
     g2 = cloud(rng, 1500, "GAUSSIAN", np.exp(rng.uniform(np.log(0.01), np.log(0.1), 1500)).astype(np.float32))[0]
     g3 = cloud(rng, 700, "SPH", np.full(700, 0.05, np.float32))[0]
     groups = [g1, g2, g3]
-    cutoff, samples, step = (40, 24), (96, 64), (0.7, 0.6)
+    cutoff, samples, step = (32, 24), (96, 64), (0.7, 0.6)
     for mode, tol in (("analytic", 1e-5), ("reference", 2e-4)):
         F, _ = gpu_spectrum(groups, cutoff, samples, step, U, V, mode, "mfma_split_f16")
         F64 = oracle.spectrum64(groups, cutoff, step, U, V, mode)
         e = oracle.rel_l2(F, F64)
         print(f"multigroup {mode}: rel-L2 {e:.3e}")
         assert e <= tol
+
+
+@pytest.mark.parametrize("basis", ["GAUSSIAN", "WENDLAND_D3_C2"])
+def test_weights_spanning_decades(basis):
+    """w ~ h^3 over two decades of h (SPH-like m/rho weights): per-bin weight scaling keeps FP16 exact enough."""
+    rng = np.random.default_rng(12)
+    n = 6000
+    h = np.exp(rng.uniform(np.log(0.002), np.log(0.2), n)).astype(np.float32)
+    w = (h.astype(np.float64) ** 3 * rng.uniform(0.5, 2.0, n)).astype(np.float32)
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    groups = [(pos, w, h, basis)]
+    cutoff, samples, step = (64, 64), (128, 128), (0.5, 0.5)
+    F, _ = gpu_spectrum(groups, cutoff, samples, step, U, V, "analytic", "mfma_split_f16")
+    F64 = oracle.spectrum64(groups, cutoff, step, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"decade weights {basis}: rel-L2 {e:.3e}")
+    assert e <= 1e-5
