@@ -7,10 +7,14 @@ namespace mv {
 // Device buffers the MFMA form keeps between renders of one config.
 struct MfmaWorkspace {
   DevBuf packets;   // per-view operand records, h-sorted
-  DevBuf partial;   // split-K FP64 partial spectra
+  DevBuf partial;   // split-K partial spectra
+  DevBuf chunks;    // int4 (bin, first step, end step, -) work items of the MFMA kernel
+  std::vector<int4> chunks_host;
   void release() {
     packets.release();
     partial.release();
+    chunks.release();
+    chunks_host.clear();
   }
 };
 

@@ -302,6 +302,8 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   bc->nb = nb;
   bc->T = T;
   bc->n_slots = n_slots;
+  bc->bin_step.resize(nb + 1);
+  for (int b = 0; b <= nb; ++b) bc->bin_step[b] = start[b] / kStep;
   if (!bc->slot_map.reserve(sizeof(int) * n_slots) || !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
       !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->inv_den.reserve(sizeof(float) * inv.size()) ||
       !bc->bininfo.reserve(sizeof(float4) * nb))

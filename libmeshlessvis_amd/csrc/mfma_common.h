@@ -3,6 +3,9 @@
 #include "mfma.h"
 #include "phi.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace mv {
 
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
@@ -14,8 +17,9 @@ constexpr int kStep = 16;          // particles per LDS stage
 constexpr int kPitch = kStep + 4;  // words per LDS row (padding against bank conflicts)
 constexpr int kTileX = 128;        // workgroup tile along x (A rows)
 constexpr int kTileY = 128;        // workgroup tile along y (B columns)
-constexpr int kThreads = 512;      // 8 waves: 2 along x (64 rows each) x 4 along y (32 columns each)
-constexpr int kFlushSteps = 32;    // fold the MFMA accumulators every 512 particles
+constexpr int kThreads = 512;      // 8 waves
+constexpr int kProducerWaves = 4;  // waves 0-3 generate operands, waves 4-7 run the MFMAs
+constexpr int kMaxChunkSteps = 1024;  // at most 16384 particles per FP32 accumulation
 constexpr float kTwoPowM32 = 2.3283064365386962890625e-10f;
 constexpr float kLog2e = 1.44269504088896340736f;
 
@@ -28,12 +32,9 @@ struct MfmaParams {
   float radius;              // uniform kind: Phi(r*radius)
   int basis, phi_mode;
   int ref_hi;                // Gaussian h_ref: 0 the bin's lower edge, 1 its upper edge
-  int tiles_x;
-  int steps_total;           // 16-particle steps over the group
-  int steps_per_split;
-  const uint16_t* step_bin;  // bin of every step (nullptr: one bin)
-  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
-  float4 bin0;               // the single bin when step_bin is nullptr
+  int tiles_x, tiles_y;
+  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|); nullptr: one bin
+  float4 bin0;               // the single bin when bininfo is nullptr
 };
 
 template <int OP>
@@ -88,6 +89,7 @@ struct BinCache {
   DevBuf nodes;     // float[nb][kMaxNodes] Chebyshev nodes in h
   DevBuf inv_den;   // float[nb][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
   DevBuf bininfo;   // float4[nb] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
+  std::vector<int> bin_step;  // host: first step of every bin, nb + 1 entries
   void release() {
     slot_map.release();
     step_bin.release();
@@ -103,6 +105,7 @@ int auto_bins(float h_min, float h_max, double ratio);
 
 // acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (the MFMA kernels' partial layout).
 bool launch_mfma_reduce(const double2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s);
+bool launch_mfma_reduce(const float2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s);
 
 // SPH / Wendland with per-particle radii: h-binned, Lagrange nodes in h (mfma_binned.hip).
 bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u,
