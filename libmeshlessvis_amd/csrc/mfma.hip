@@ -53,10 +53,14 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __r
   const float4 bi = p.bininfo ? p.bininfo[ch.x] : p.bin0;
   const float hh_ref = p.ref_hi ? bi.y : bi.x;
 
+  // Waves w and w+4 of a workgroup share a SIMD on gfx950, so every SIMD runs one producer
+  // and one consumer wave and the MFMAs of one overlap the VALU work of the other.
+  const int role_idx = wave & (kProducerWaves - 1);
   if (wave < kProducerWaves) {
     // ---- producer: operands of step s+1 while the consumers multiply step s ----
-    const int gp = tid & (kStep - 1);  // particle of the step
-    const int gg = tid >> 4;           // 0..15: rows / columns gg + 16m
+    const int ptid = role_idx * 64 + lane;
+    const int gp = ptid & (kStep - 1);  // particle of the step
+    const int gg = ptid >> 4;           // 0..15: rows / columns gg + 16m
     uint32_t ux[8], uy[8];
     float ca[8], cb[8];
 #pragma unroll
@@ -117,7 +121,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __r
   }
 
   // ---- consumer: a 64 (x) x 64 (y) quarter of the tile, four 32x32 blocks ----
-  const int cw = wave - kProducerWaves;
+  const int cw = role_idx;
   const int wx = cw & 1;
   const int wy = cw >> 1;
   f32x16_t acc[2][2][2];  // [row block][column block][re, im]
