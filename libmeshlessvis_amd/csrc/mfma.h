@@ -1,30 +1,70 @@
-// Split-precision MFMA form of the direct sum (see mfma.hip).
+// MFMA (matrix-core) forms of the direct sum: host interface.
 #pragma once
 #include "internal.h"
 
+#include <tuple>
+#include <vector>
+
 namespace mv {
 
-// Device buffers the MFMA form keeps between renders of one config.
+enum MfmaKind { KIND_UNIFORM = 0, KIND_GAUSS = 1, KIND_BINNED = 2 };
+
+struct MfmaJob;
+struct MfmaItem;
+struct BinCache;
+
+// One (view, group) pair to sum, and where its unscaled spectrum goes:
+// acc[2cx*cy] (+)= sum_i w_i Phi(r h_i) exp(+2 pi i (x a_i + y b_i)).
+struct MfmaJobSpec {
+  const Group* g = nullptr;
+  GroupInfo info;
+  int n = 0;
+  ViewParams vp{};
+  double3 u{}, v{};
+  const VisConfig* c = nullptr;
+  double2* acc = nullptr;
+  bool first = true;
+};
+
+// Host plan of one job.
+struct MfmaPlanned;
+
+// Work tables of a prepared batch (device pointers into the workspace).
+struct MfmaBatch {
+  std::vector<MfmaPlanned> plans;
+  std::vector<std::tuple<int, int, int>> keys;  // (kind, basis, nodes) of every launch
+  std::vector<int> job_begin, item_begin;
+  const MfmaJob* jobs = nullptr;
+  const MfmaItem* items = nullptr;
+  int* counters = nullptr;
+};
+
+// Device buffers the MFMA forms keep between calls.
 struct MfmaWorkspace {
-  DevBuf packets;   // per-view operand records, h-sorted
-  DevBuf partial;   // split-K partial spectra
-  DevBuf chunks;    // int4 (bin, first step, end step, -) work items of the MFMA kernel
-  std::vector<int4> chunks_host;
+  DevBuf packets;   // per-view packets of every job, bin-ordered
+  DevBuf partial;   // float2 partial spectra of every work chunk
+  DevBuf tables;    // MfmaJob[] then MfmaItem[]
+  DevBuf counters;  // one work-queue counter per launch
+  std::vector<char> tables_host;
+  MfmaBatch batch;
   void release() {
     packets.release();
     partial.release();
-    chunks.release();
-    chunks_host.clear();
+    tables.release();
+    counters.release();
   }
 };
 
-// Adds (assigns when first) the group's unscaled sum
-// sum_i w_i Phi(r h_i) exp(+2 pi i (x a_i + y b_i)) into acc[2cx*cy].
-bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u, double3 v,
-                         const VisConfig* c, bool first, hipStream_t s, MfmaWorkspace* ws, double2* acc,
-                         EventPair* ev);
+// Plans (bins, packet slots, work items) and uploads the work tables of a batch.
+bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, MfmaBatch* bt);
+// Launches a prepared batch: packet prep, persistent MFMA kernels, reductions (graph-capturable).
+bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev);
+// prepare + enqueue.
+bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
 
-// Drops what the MFMA form cached about a group (called on unregister).
+// Drops what the MFMA forms cached about a group (called on unregister).
 void mfma_forget_group(const void* key);
 
 }  // namespace mv
+
+#include "mfma_plan.h"

@@ -4,212 +4,226 @@
 // Phi(r h_i) factors into an x part and a y part the half-plane sum is a
 // complex matrix product over the particles i:
 //
-//     S(x, y) = E(x, y) * sum_i A(x, i) B(i, y)
-//     A(x, i) = alpha_i(x) exp(+2 pi i x a_i),  B(i, y) = w_i' beta_i(y) exp(+2 pi i y b_i)
+//     S(x, y) = E_b(x, y) * sum_{i in b} A(x, i) B(i, y)        (b: h-bin of i)
+//     A(x, i) = alpha_i(x) exp(+2 pi i x a_i),  B(i, y) = (w_i / W_b) beta_i(y) exp(+2 pi i y b_i)
 //
-//  * uniform radius h (or no radii, h = 1), any basis:
-//      alpha = beta = 1, E = Phi(r h) * wmax
-//  * Gaussian with per-particle radii (exact, no binning: exp(s pi h^2 r^2)
-//    = exp(s pi h^2 fu^2) exp(s pi h^2 fv^2), s = -1 analytic, +1 reference):
-//      alpha_i(x) = exp(s pi (h_i^2 - h_ref^2) fu^2), beta_i(y) likewise with fv,
-//      E = exp(s pi h_ref^2 r^2) * wmax, h_ref the group's min (s=-1) or max (s=+1)
-//      radius so that |alpha|, |beta| <= 1;
-//  w_i' = w_i / wmax keeps |B| <= 1 (FP16 range).
+//  * uniform radius h (or no radii, h = 1), any basis, one bin:
+//      alpha = beta = 1, E = Phi(r h) * W
+//  * Gaussian with per-particle radii (exact, no interpolation: exp(s pi h^2 r^2)
+//    = exp(s pi h^2 fu^2) exp(s pi h^2 fv^2), s = -1 analytic, +1 reference),
+//    log-spaced h-bins of ratio 1.25:
+//      alpha_i(x) = exp(s pi (h_i^2 - h_b^2) fu^2), beta_i(y) likewise with fv,
+//      E_b = exp(s pi h_b^2 r^2) * W_b, h_b the bin's lower (s=-1) or upper (s=+1)
+//      edge so that |alpha|, |beta| <= 1;
+//  W_b = the bin's largest |w| keeps |B| <= 1 and FP16 operands out of the
+//  subnormal range (weights correlate with h).
 //
-// Kernel (gfx950, wave64, v_mfma_f32_32x32x16_{f16,bf16}):
-//  * a workgroup is 8 waves and owns a 128 (x) x 128 (y) complex output tile;
-//    a wave owns 64 x 32 (two 32x32 blocks, real and imaginary accumulators);
+// Kernel (gfx950, wave64, v_mfma_f32_32x32x16_{f16,bf16}), persistent:
+//  * the workgroups pull work items (job, 128x128 output tile, range of
+//    16-particle steps inside one bin) from an atomic counter, items longest
+//    first; one launch covers every view and group of a render batch;
+//  * 8 waves: waves 0-3 generate operands, waves 4-7 run the MFMAs.  Waves w
+//    and w+4 share a SIMD on gfx950, so each SIMD overlaps one wave's VALU
+//    work with the other's matrix work;
+//  * operands are generated per 16-particle step into double-buffered LDS as
+//    packed half pairs: phase from the particle's fixed-point phase steps
+//    (exact modular arithmetic), v_sin/v_cos in revolutions, the Gaussian
+//    factor, then hi = rnd(v), lo = rnd(v - hi) (split forms); one barrier
+//    per step;
 //  * the complex product runs as real MFMAs with K = (re, im) pairs of 8
 //    particles: acc_re += [Ar Ai].[Br -Bi], acc_im += [Ar Ai].[Bi Br]; the B
-//    operand forms are a sign flip and a 16-bit rotation of the packed (Br, Bi);
-//  * operands are generated on the fly per 16-particle step into double-
-//    buffered LDS as packed half pairs: phase from the particle's fixed-point
-//    phase steps (exact modular arithmetic), v_sin/v_cos in revolutions, the
-//    Gaussian factor, then hi = rnd(v), lo = rnd(v - hi) (split forms);
-//    one barrier per step, generation of step s+1 overlaps the MFMAs of step s;
-//  * split products: hi*hi + hi*lo + lo*hi into the same FP32 accumulators;
-//  * accumulation: MFMA FP32 accumulators are folded into a second FP32 set
-//    every 512 particles; split-K partials (FP64) are reduced in order.
+//    forms are a sign flip and a 16-bit rotation of the packed (Br, Bi);
+//    split products hi*hi + hi*lo + lo*hi share the FP32 accumulators;
+//  * an item spans at most 16384 particles of FP32 accumulation; its partial
+//    tile (FP32) is reduced with the job's other items in FP64, in order.
 #include "mfma_common.h"
+
+#include <cstring>
+#include <tuple>
 
 namespace mv {
 
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
-__global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const Packet* __restrict__ pk, MfmaParams p,
-                                                                 const int4* __restrict__ chunks,
-                                                                 float2* __restrict__ partial) {
+__global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
+                                                                 const MfmaItem* __restrict__ items, MfmaShared sh,
+                                                                 int* __restrict__ counter) {
   // [buffer][A_hi, A_lo, B_hi, B_lo][row * kPitch + particle], one packed (re, im) pair per word
   __shared__ uint32_t lds[2][4][kTileX * kPitch];
+  __shared__ int next_item;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int tiles = p.tiles_x * p.tiles_y;
-  const int tile = blockIdx.x % tiles;
-  const int chunk = blockIdx.x / tiles;
-  const int j0 = (tile % p.tiles_x) * kTileX;
-  const int y0 = (tile / p.tiles_x) * kTileY;
-  const int4 ch = chunks[chunk];  // (bin, first step, end step, -)
-  const float4 bi = p.bininfo ? p.bininfo[ch.x] : p.bin0;
-  const float hh_ref = p.ref_hi ? bi.y : bi.x;
-
   // Waves w and w+4 of a workgroup share a SIMD on gfx950, so every SIMD runs one producer
   // and one consumer wave and the MFMAs of one overlap the VALU work of the other.
   const int role_idx = wave & (kProducerWaves - 1);
-  if (wave < kProducerWaves) {
-    // ---- producer: operands of step s+1 while the consumers multiply step s ----
-    const int ptid = role_idx * 64 + lane;
-    const int gp = ptid & (kStep - 1);  // particle of the step
-    const int gg = ptid >> 4;           // 0..15: rows / columns gg + 16m
-    uint32_t ux[8], uy[8];
-    float ca[8], cb[8];
-#pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      const int x = j0 + gg + 16 * m - p.cx + 1;
-      const int y = y0 + gg + 16 * m;
-      ux[m] = static_cast<uint32_t>(x);
-      uy[m] = static_cast<uint32_t>(y);
-      const float fu = p.sx * static_cast<float>(x);
-      const float fv = p.sy * static_cast<float>(y);
-      ca[m] = p.sigma_pi_log2e * fu * fu;
-      cb[m] = p.sigma_pi_log2e * fv * fv;
-    }
-    auto generate = [&](int step, int buf) {
-      const Packet q = pk[static_cast<size_t>(step) * kStep + gp];
-      const float dh = q.h * q.h - hh_ref;
-      const float wq = q.w * bi.w;
+  const int two_cx = 2 * sh.cx;
+
+  for (;;) {
+    if (tid == 0) next_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next_item;
+    __syncthreads();
+    if (it >= sh.n_items) break;
+    const MfmaItem item = items[it];
+    const MfmaJob job = jobs[item.job];
+    const int j0 = (item.tile % job.tiles_x) * kTileX;
+    const int y0 = (item.tile / job.tiles_x) * kTileY;
+    const float4 bi = job.bininfo ? job.bininfo[item.bin] : job.bin0;
+    const float hh_ref = sh.ref_hi ? bi.y : bi.x;
+
+    if (wave < kProducerWaves) {
+      // ---- producer: operands of step s+1 while the consumers multiply step s ----
+      const int ptid = role_idx * 64 + lane;
+      const int gp = ptid & (kStep - 1);  // particle of the step
+      const int gg = ptid >> 4;           // 0..15: rows / columns gg + 16m
+      uint32_t ux[8], uy[8];
+      float ca[8], cb[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const int o = (gg + 16 * m) * kPitch + gp;
-        {  // A(x, i)
-          const uint32_t P = ux[m] * q.pu;
-          const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-          float amp = 1.0f;
-          if (GAUSS) amp = __builtin_amdgcn_exp2f(fminf(ca[m] * dh, 0.0f));
-          const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
-          const uint32_t hi = pack2<OP>(re, im);
-          lds[buf][0][o] = hi;
-          if (SPLIT) {
-            float hr, hm;
-            unpack2<OP>(hi, &hr, &hm);
-            lds[buf][1][o] = pack2<OP>(re - hr, im - hm);
-          }
-        }
-        {  // B(i, y)
-          const uint32_t P = uy[m] * q.pv;
-          const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-          float amp = wq;
-          if (GAUSS) amp *= __builtin_amdgcn_exp2f(fminf(cb[m] * dh, 0.0f));
-          const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
-          const uint32_t hi = pack2<OP>(re, im);
-          lds[buf][2][o] = hi;
-          if (SPLIT) {
-            float hr, hm;
-            unpack2<OP>(hi, &hr, &hm);
-            lds[buf][3][o] = pack2<OP>(re - hr, im - hm);
-          }
-        }
+        const int x = j0 + gg + 16 * m - sh.cx + 1;
+        const int y = y0 + gg + 16 * m;
+        ux[m] = static_cast<uint32_t>(x);
+        uy[m] = static_cast<uint32_t>(y);
+        const float fu = sh.sx * static_cast<float>(x);
+        const float fv = sh.sy * static_cast<float>(y);
+        ca[m] = sh.sigma_pi_log2e * fu * fu;
+        cb[m] = sh.sigma_pi_log2e * fv * fv;
       }
-    };
-    if (ch.y < ch.z) generate(ch.y, 0);
-    __syncthreads();
-    for (int s = ch.y; s < ch.z; ++s) {
-      if (s + 1 < ch.z) generate(s + 1, (s - ch.y + 1) & 1);
+      auto generate = [&](int step, int buf) {
+        const Packet q = job.packets[static_cast<size_t>(step) * kStep + gp];
+        const float dh = q.h * q.h - hh_ref;
+        const float wq = q.w * bi.w;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+          const int o = (gg + 16 * m) * kPitch + gp;
+          {  // A(x, i)
+            const uint32_t P = ux[m] * q.pu;
+            const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+            float amp = 1.0f;
+            if (GAUSS) amp = __builtin_amdgcn_exp2f(fminf(ca[m] * dh, 0.0f));
+            const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
+            const uint32_t hi = pack2<OP>(re, im);
+            lds[buf][0][o] = hi;
+            if (SPLIT) {
+              float hr, hm;
+              unpack2<OP>(hi, &hr, &hm);
+              lds[buf][1][o] = pack2<OP>(re - hr, im - hm);
+            }
+          }
+          {  // B(i, y)
+            const uint32_t P = uy[m] * q.pv;
+            const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+            float amp = wq;
+            if (GAUSS) amp *= __builtin_amdgcn_exp2f(fminf(cb[m] * dh, 0.0f));
+            const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
+            const uint32_t hi = pack2<OP>(re, im);
+            lds[buf][2][o] = hi;
+            if (SPLIT) {
+              float hr, hm;
+              unpack2<OP>(hi, &hr, &hm);
+              lds[buf][3][o] = pack2<OP>(re - hr, im - hm);
+            }
+          }
+        }
+      };
+      if (item.s0 < item.s1) generate(item.s0, 0);
       __syncthreads();
-    }
-    return;
-  }
-
-  // ---- consumer: a 64 (x) x 64 (y) quarter of the tile, four 32x32 blocks ----
-  const int cw = role_idx;
-  const int wx = cw & 1;
-  const int wy = cw >> 1;
-  f32x16_t acc[2][2][2];  // [row block][column block][re, im]
-#pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-    for (int cb = 0; cb < 2; ++cb) acc[rb][cb][0] = acc[rb][cb][1] = (f32x16_t){};
-
-  __syncthreads();
-  for (int s = ch.y; s < ch.z; ++s) {
-    const int buf = (s - ch.y) & 1;
-#pragma unroll
-    for (int kk = 0; kk < kStep / 8; ++kk) {
-      const int pofs = kk * 8 + (lane >> 5) * 4;
-      u32x4_t bre_hi[2], bim_hi[2], bre_lo[2], bim_lo[2];
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb) {
-        const int col = wy * 64 + cb * 32 + (lane & 31);
-        const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&lds[buf][2][col * kPitch + pofs]);
-        bre_hi[cb] = negate_imag(bhi);
-        bim_hi[cb] = swap_halves(bhi);
-        if (SPLIT) {
-          const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&lds[buf][3][col * kPitch + pofs]);
-          bre_lo[cb] = negate_imag(blo);
-          bim_lo[cb] = swap_halves(blo);
-        }
+      for (int s = item.s0; s < item.s1; ++s) {
+        if (s + 1 < item.s1) generate(s + 1, (s - item.s0 + 1) & 1);
+        __syncthreads();
       }
+      continue;
+    }
+
+    // ---- consumer: a 64 (x) x 64 (y) quarter of the tile, four 32x32 blocks ----
+    const int wx = role_idx & 1;
+    const int wy = role_idx >> 1;
+    f32x16_t acc[2][2][2];  // [row block][column block][re, im]
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-        const int row = wx * 64 + rb * 32 + (lane & 31);
-        const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&lds[buf][0][row * kPitch + pofs]);
-        u32x4_t alo;
-        if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&lds[buf][1][row * kPitch + pofs]);
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) acc[rb][cb][0] = acc[rb][cb][1] = (f32x16_t){};
+
+    __syncthreads();
+    for (int s = item.s0; s < item.s1; ++s) {
+      const int buf = (s - item.s0) & 1;
+#pragma unroll
+      for (int kk = 0; kk < kStep / 8; ++kk) {
+        const int pofs = kk * 8 + (lane >> 5) * 4;
+        u32x4_t bre_hi[2], bim_hi[2], bre_lo[2], bim_lo[2];
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
-          acc[rb][cb][0] = mfma<OP>(ahi, bre_hi[cb], acc[rb][cb][0]);
-          acc[rb][cb][1] = mfma<OP>(ahi, bim_hi[cb], acc[rb][cb][1]);
+          const int col = wy * 64 + cb * 32 + (lane & 31);
+          const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&lds[buf][2][col * kPitch + pofs]);
+          bre_hi[cb] = negate_imag(bhi);
+          bim_hi[cb] = swap_halves(bhi);
           if (SPLIT) {
-            acc[rb][cb][0] = mfma<OP>(ahi, bre_lo[cb], acc[rb][cb][0]);
-            acc[rb][cb][1] = mfma<OP>(ahi, bim_lo[cb], acc[rb][cb][1]);
-            acc[rb][cb][0] = mfma<OP>(alo, bre_hi[cb], acc[rb][cb][0]);
-            acc[rb][cb][1] = mfma<OP>(alo, bim_hi[cb], acc[rb][cb][1]);
+            const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&lds[buf][3][col * kPitch + pofs]);
+            bre_lo[cb] = negate_imag(blo);
+            bim_lo[cb] = swap_halves(blo);
+          }
+        }
+#pragma unroll
+        for (int rb = 0; rb < 2; ++rb) {
+          const int row = wx * 64 + rb * 32 + (lane & 31);
+          const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&lds[buf][0][row * kPitch + pofs]);
+          u32x4_t alo;
+          if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&lds[buf][1][row * kPitch + pofs]);
+#pragma unroll
+          for (int cb = 0; cb < 2; ++cb) {
+            acc[rb][cb][0] = mfma<OP>(ahi, bre_hi[cb], acc[rb][cb][0]);
+            acc[rb][cb][1] = mfma<OP>(ahi, bim_hi[cb], acc[rb][cb][1]);
+            if (SPLIT) {
+              acc[rb][cb][0] = mfma<OP>(ahi, bre_lo[cb], acc[rb][cb][0]);
+              acc[rb][cb][1] = mfma<OP>(ahi, bim_lo[cb], acc[rb][cb][1]);
+              acc[rb][cb][0] = mfma<OP>(alo, bre_hi[cb], acc[rb][cb][0]);
+              acc[rb][cb][1] = mfma<OP>(alo, bim_hi[cb], acc[rb][cb][1]);
+            }
           }
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
-  }
 
-  // Epilogue: partial[chunk][j][y] = E_b(x, y) * acc,
-  // E_b = exp(s pi h_ref^2 r^2) W_b (Gaussian) or Phi(r h) W_b (uniform radius).
-  const int two_cx = 2 * p.cx;
+    // Epilogue: partial[chunk][j][y] = E_b(x, y) * acc,
+    // E_b = exp(s pi h_b^2 r^2) W_b (Gaussian) or Phi(r h) W (uniform radius).
 #pragma unroll
-  for (int cb = 0; cb < 2; ++cb) {
-    const int y = y0 + wy * 64 + cb * 32 + (lane & 31);
-    if (y >= p.cy) continue;
-    const float fv = p.sy * static_cast<float>(y);
+    for (int cb = 0; cb < 2; ++cb) {
+      const int y = y0 + wy * 64 + cb * 32 + (lane & 31);
+      if (y >= sh.cy) continue;
+      const float fv = sh.sy * static_cast<float>(y);
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb) {
+      for (int rb = 0; rb < 2; ++rb) {
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int j = j0 + wx * 64 + rb * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-        if (j >= two_cx) continue;
-        const float fu = p.sx * static_cast<float>(j - p.cx + 1);
-        const float r2 = fu * fu + fv * fv;
-        float e;
-        if (GAUSS)
-          e = __builtin_amdgcn_exp2f(p.sigma_pi_log2e * hh_ref * r2) * bi.z;
-        else
-          e = phi<BASIS, MODE>(sqrtf(r2) * p.radius) * bi.z;
-        partial[(static_cast<size_t>(chunk) * two_cx + j) * p.cy + y] =
-            make_float2(e * acc[rb][cb][0][i], e * acc[rb][cb][1][i]);
+        for (int i = 0; i < 16; ++i) {
+          const int j = j0 + wx * 64 + rb * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+          if (j >= two_cx) continue;
+          const float fu = sh.sx * static_cast<float>(j - sh.cx + 1);
+          const float r2 = fu * fu + fv * fv;
+          float e;
+          if (GAUSS)
+            e = __builtin_amdgcn_exp2f(sh.sigma_pi_log2e * hh_ref * r2) * bi.z;
+          else
+            e = phi<BASIS, MODE>(sqrtf(r2) * job.radius) * bi.z;
+          job.partial[(static_cast<size_t>(item.chunk) * two_cx + j) * sh.cy + y] =
+              make_float2(e * acc[rb][cb][0][i], e * acc[rb][cb][1][i]);
+        }
       }
     }
   }
 }
 
 // acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (in order, FP64), x0 the reference column of x = j - cx + 1.
-template <class PT>
-__global__ __launch_bounds__(256) void mfma_reduce_kernel(const PT* __restrict__ partial, int ksplit, int cx, int cy,
-                                                          int first, double2* __restrict__ acc) {
+__global__ __launch_bounds__(256) void mfma_reduce_kernel(const float2* __restrict__ partial, int nchunks, int cx,
+                                                          int cy, int first, double2* __restrict__ acc) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int two_cx = 2 * cx;
   const int nfreq = two_cx * cy;
   if (f >= nfreq) return;
   double re = 0.0, im = 0.0;
-  for (int k = 0; k < ksplit; ++k) {
-    const PT v = partial[static_cast<size_t>(k) * nfreq + f];
+  for (int k = 0; k < nchunks; ++k) {
+    const float2 v = partial[static_cast<size_t>(k) * nfreq + f];
     re += static_cast<double>(v.x);
     im += static_cast<double>(v.y);
   }
@@ -225,140 +239,267 @@ __global__ __launch_bounds__(256) void mfma_reduce_kernel(const PT* __restrict__
   acc[o] = make_double2(re, im);
 }
 
+__global__ void zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ------------------------------------------------------------------ host ---
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
-static void launch_k(dim3 grid, hipStream_t s, const Packet* pk, const MfmaParams& p, const int4* chunks,
-                     float2* partial) {
-  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE>), grid, dim3(kThreads), 0, s, pk, p, chunks,
-                     partial);
+static void launch_sum(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                       int* counter) {
+  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE>), dim3(grid), dim3(kThreads), 0, s, jobs, items,
+                     sh, counter);
 }
 
 template <int OP, bool SPLIT>
-static void launch_kind(dim3 grid, hipStream_t s, const Packet* pk, const MfmaParams& p, const int4* chunks,
-                        float2* partial, bool gauss) {
-  if (gauss) {
-    launch_k<OP, SPLIT, true, GAUSSIAN, VIS_PHI_ANALYTIC>(grid, s, pk, p, chunks, partial);
+static void launch_sum_kind(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                            int* counter, int kind, int basis, int mode) {
+  if (kind == KIND_GAUSS) {
+    launch_sum<OP, SPLIT, true, GAUSSIAN, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter);
     return;
   }
-  const bool an = p.phi_mode == VIS_PHI_ANALYTIC;
-  switch (p.basis) {
+  const bool an = mode == VIS_PHI_ANALYTIC;
+  switch (basis) {
     case SPH:
-      an ? launch_k<OP, SPLIT, false, SPH, VIS_PHI_ANALYTIC>(grid, s, pk, p, chunks, partial)
-         : launch_k<OP, SPLIT, false, SPH, VIS_PHI_REFERENCE>(grid, s, pk, p, chunks, partial);
+      an ? launch_sum<OP, SPLIT, false, SPH, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+         : launch_sum<OP, SPLIT, false, SPH, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
       break;
     case GAUSSIAN:
-      an ? launch_k<OP, SPLIT, false, GAUSSIAN, VIS_PHI_ANALYTIC>(grid, s, pk, p, chunks, partial)
-         : launch_k<OP, SPLIT, false, GAUSSIAN, VIS_PHI_REFERENCE>(grid, s, pk, p, chunks, partial);
+      an ? launch_sum<OP, SPLIT, false, GAUSSIAN, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+         : launch_sum<OP, SPLIT, false, GAUSSIAN, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
       break;
     default:
-      an ? launch_k<OP, SPLIT, false, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, pk, p, chunks, partial)
-         : launch_k<OP, SPLIT, false, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, pk, p, chunks, partial);
+      an ? launch_sum<OP, SPLIT, false, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+         : launch_sum<OP, SPLIT, false, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
   }
 }
 
-bool mfma_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u, double3 v,
-                         const VisConfig* c, bool first, hipStream_t s, MfmaWorkspace* ws, double2* acc,
-                         EventPair* ev) {
-  const bool uniform = info.uniform_radius;
-  const bool gauss = !uniform && vp.basis == GAUSSIAN;
-  if (!uniform && !gauss) return mfma_binned_group_spectrum(g, info, n, vp, u, v, c, first, s, ws, acc, ev);
-  const int nfreq = 2 * vp.cx * vp.cy;
+static int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+      cus = 256;
+  }
+  return cus;
+}
 
-  MfmaParams p;
-  p.cx = vp.cx;
-  p.cy = vp.cy;
-  p.sx = vp.sx;
-  p.sy = vp.sy;
-  p.basis = vp.basis;
-  p.phi_mode = vp.phi_mode;
-  const float sigma = (vp.phi_mode == VIS_PHI_REFERENCE) ? 1.0f : -1.0f;
-  p.sigma_pi_log2e = gauss ? sigma * kPi * kLog2e : 0.0f;
-  p.ref_hi = sigma > 0.0f ? 1 : 0;
-  p.radius = info.has_radii ? info.radius : 1.0f;
-  p.tiles_x = (2 * vp.cx + kTileX - 1) / kTileX;
-  p.tiles_y = (vp.cy + kTileY - 1) / kTileY;
-  std::vector<int> bin_step;  // first step of every bin
-  if (gauss) {
-    // Bins of h ratio 1.25: per-bin weight scale and reference radius.
-    const int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.25);
+int mfma_kind(const GroupInfo& info, int basis) {
+  if (info.uniform_radius) return KIND_UNIFORM;
+  return basis == GAUSSIAN ? KIND_GAUSS : KIND_BINNED;
+}
+
+// Bins, packet slots and work items of one job (host side; builds the group's
+// bins on first use).
+bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, MfmaPlanned* out) {
+  const VisConfig* c = spec.c;
+  const GroupInfo& info = spec.info;
+  out->kind = mfma_kind(info, spec.vp.basis);
+  out->basis = spec.vp.basis;
+  out->T = 1;
+  out->bc = nullptr;
+  out->items.clear();
+  MfmaJob& job = out->job;
+  job = MfmaJob{};
+  job.radius = info.has_radii ? info.radius : 1.0f;
+  std::vector<int> bin_step;
+  if (out->kind == KIND_UNIFORM) {
+    out->n_slots = std::max(kStep, ((spec.n + kStep - 1) / kStep) * kStep);
+    job.bin0 = make_float4(0.0f, 0.0f, info.w_absmax, info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f);
+    bin_step = {0, out->n_slots / kStep};
+  } else {
+    int nb, T = 1;
+    if (out->kind == KIND_GAUSS) {
+      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.25);
+    } else {
+      T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 3;
+      if (T < 2 || T > kMaxNodes) {
+        set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
+        return false;
+      }
+      // Defaults: 3 Chebyshev nodes per bin of ratio 1.04 (interpolation error < 3e-6 of Phi(0)).
+      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, T == 3 ? 1.04 : 1.1);
+    }
     if (nb > 65535) {
       set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
       return false;
     }
-    const BinCache* bc = bin_cache(g, info, n, nb, 1, s);
+    const BinCache* bc = bin_cache(*spec.g, info, spec.n, nb, T, s);
     if (!bc) return false;
-    if (!ws->packets.reserve(sizeof(Packet) * bc->n_slots)) return false;
-    if (!hip_ok(launch_prep_packets_mapped(g.d_constraints, g.d_radii, bc->slot_map.as<int>(), bc->n_slots,
-                                           c->step_size.x, c->step_size.y, u, v, ws->packets.as<Packet>(), s),
-                "prep_packets_mapped launch"))
-      return false;
-    p.bininfo = bc->bininfo.as<float4>();
-    p.bin0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    out->bc = bc;
+    out->T = T;
+    out->n_slots = bc->n_slots;
+    job.bininfo = bc->bininfo.as<float4>();
+    job.step_bin = bc->step_bin.as<uint16_t>();
+    job.nodes = bc->nodes.as<float>();
+    job.inv_den = bc->inv_den.as<float>();
     bin_step = bc->bin_step;
-  } else {
-    const int n_padded = ((n + 255) / 256) * 256;
-    if (!ws->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
-    if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
-                                    ws->packets.as<Packet>(), s),
-                "prep_packets launch"))
-      return false;
-    p.bininfo = nullptr;
-    p.bin0 = make_float4(0.0f, 0.0f, info.w_absmax, info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f);
-    bin_step = {0, n_padded / kStep};
   }
-  // Work items: (bin, step range) chunks of at most kMaxChunkSteps steps, sized so that
-  // tiles x chunks fills the chip several times over.
-  const int tiles = p.tiles_x * p.tiles_y;
+  // Work items.  Sum kernel: bin-aligned step ranges of at most kMaxChunkSteps; binned
+  // kernel: ranges may cross bins (it folds at bin changes).  Sized so that a single
+  // render still fills the chip several times over.
+  const bool binned = out->kind == KIND_BINNED;
+  const int tx = binned ? kBinnedTileX : kTileX, ty = binned ? kBinnedTileY : kTileY;
+  job.tiles_x = (2 * spec.vp.cx + tx - 1) / tx;
+  const int tiles = job.tiles_x * ((spec.vp.cy + ty - 1) / ty);
   const long long total_steps = bin_step.back();
-  const int chunk_steps = static_cast<int>(
-      std::min<long long>(kMaxChunkSteps, std::max<long long>(32, total_steps * tiles / 2048)));
-  std::vector<int4> chunks;
-  for (size_t b = 0; b + 1 < bin_step.size(); ++b)
-    for (int s0 = bin_step[b]; s0 < bin_step[b + 1]; s0 += chunk_steps)
-      chunks.push_back(make_int4(static_cast<int>(b), s0, std::min(bin_step[b + 1], s0 + chunk_steps), 0));
-  if (chunks.empty()) chunks.push_back(make_int4(0, 0, 0, 0));
-  const int nchunks = static_cast<int>(chunks.size());
-  const bool same = ws->chunks_host.size() == chunks.size() &&
-                    std::equal(chunks.begin(), chunks.end(), ws->chunks_host.begin(), [](const int4& a, const int4& b) {
-                      return a.x == b.x && a.y == b.y && a.z == b.z;
-                    });
-  if (!same) {
-    if (!ws->chunks.reserve(sizeof(int4) * nchunks)) return false;
-    if (!hip_ok(hipStreamSynchronize(s), "synchronize") ||
-        !hip_ok(hipMemcpy(ws->chunks.ptr, chunks.data(), sizeof(int4) * nchunks, hipMemcpyHostToDevice), "upload"))
-      return false;
-    ws->chunks_host = chunks;
+  const int cap = binned ? 4096 : kMaxChunkSteps;
+  const int floor_steps = binned ? 128 : 32;
+  const int chunk_steps =
+      static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, total_steps * tiles / 2048)));
+  int chunk = 0;
+  auto add_range = [&](int bin, int s0, int s1) {
+    for (int t = 0; t < tiles; ++t) {
+      MfmaItem it{};
+      it.tile = t;
+      it.s0 = s0;
+      it.s1 = s1;
+      it.bin = bin;
+      it.chunk = chunk;
+      out->items.push_back(it);
+    }
+    ++chunk;
+  };
+  if (binned) {
+    for (long long s0 = 0; s0 < total_steps; s0 += chunk_steps)
+      add_range(0, static_cast<int>(s0), static_cast<int>(std::min<long long>(total_steps, s0 + chunk_steps)));
+  } else {
+    for (size_t b = 0; b + 1 < bin_step.size(); ++b)
+      for (int s0 = bin_step[b]; s0 < bin_step[b + 1]; s0 += chunk_steps)
+        add_range(static_cast<int>(b), s0, std::min(bin_step[b + 1], s0 + chunk_steps));
   }
-  if (!ws->partial.reserve(sizeof(float2) * static_cast<size_t>(nchunks) * nfreq)) return false;
+  if (chunk == 0) add_range(0, 0, 0);
+  out->nchunks = chunk;
+  return true;
+}
 
-  const dim3 grid(tiles * nchunks);
+// Host half of a batch: plans, buffer sizes, work tables uploaded (synchronous).
+bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, MfmaBatch* bt) {
+  bt->plans.assign(specs.size(), MfmaPlanned{});
+  bt->keys.clear();
+  bt->job_begin.clear();
+  bt->item_begin.clear();
+  if (specs.empty()) return true;
+  size_t slots = 0, parts = 0;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    if (!mfma_plan(specs[k], s, &bt->plans[k])) return false;
+    bt->plans[k].pk_off = slots;
+    bt->plans[k].part_off = parts;
+    slots += bt->plans[k].n_slots;
+    parts += static_cast<size_t>(bt->plans[k].nchunks) * 2 * specs[k].vp.cx * specs[k].vp.cy;
+  }
+  if (!ws->packets.reserve(sizeof(Packet) * slots) || !ws->partial.reserve(sizeof(float2) * std::max<size_t>(parts, 1)))
+    return false;
+  for (auto& pl : bt->plans) {
+    pl.job.packets = ws->packets.as<Packet>() + pl.pk_off;
+    pl.job.partial = ws->partial.as<float2>() + pl.part_off;
+  }
+  // One persistent launch per kernel variant present (kind, basis, nodes).
+  auto key_of = [](const MfmaPlanned& pl) {
+    return std::make_tuple(pl.kind, pl.kind == KIND_GAUSS ? static_cast<int>(GAUSSIAN) : pl.basis, pl.T);
+  };
+  for (auto& pl : bt->plans)
+    if (std::find(bt->keys.begin(), bt->keys.end(), key_of(pl)) == bt->keys.end()) bt->keys.push_back(key_of(pl));
+  std::vector<MfmaJob> jobs;
+  std::vector<MfmaItem> items;
+  for (auto& key : bt->keys) {
+    bt->job_begin.push_back(static_cast<int>(jobs.size()));
+    bt->item_begin.push_back(static_cast<int>(items.size()));
+    std::vector<MfmaItem> its;
+    for (auto& pl : bt->plans) {
+      if (key_of(pl) != key) continue;
+      const int jid = static_cast<int>(jobs.size()) - bt->job_begin.back();
+      jobs.push_back(pl.job);
+      for (MfmaItem it : pl.items) {
+        it.job = jid;
+        its.push_back(it);
+      }
+    }
+    // Longest ranges first: the atomic queue then balances the tail.
+    std::stable_sort(its.begin(), its.end(),
+                     [](const MfmaItem& a, const MfmaItem& b) { return (a.s1 - a.s0) > (b.s1 - b.s0); });
+    items.insert(items.end(), its.begin(), its.end());
+  }
+  bt->job_begin.push_back(static_cast<int>(jobs.size()));
+  bt->item_begin.push_back(static_cast<int>(items.size()));
+  const size_t jbytes = sizeof(MfmaJob) * jobs.size(), ibytes = sizeof(MfmaItem) * items.size();
+  if (!ws->tables.reserve(jbytes + ibytes) || !ws->counters.reserve(sizeof(int) * bt->keys.size())) return false;
+  ws->tables_host.resize(jbytes + ibytes);
+  std::memcpy(ws->tables_host.data(), jobs.data(), jbytes);
+  std::memcpy(ws->tables_host.data() + jbytes, items.data(), ibytes);
+  if (!hip_ok(hipStreamSynchronize(s), "synchronize") ||
+      !hip_ok(hipMemcpy(ws->tables.ptr, ws->tables_host.data(), jbytes + ibytes, hipMemcpyHostToDevice),
+              "upload work tables"))
+    return false;
+  bt->jobs = reinterpret_cast<const MfmaJob*>(ws->tables.ptr);
+  bt->items = reinterpret_cast<const MfmaItem*>(static_cast<char*>(ws->tables.ptr) + jbytes);
+  bt->counters = ws->counters.as<int>();
+  return true;
+}
+
+// Device half: kernel launches only (safe to capture into a HIP graph).
+bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
+  if (specs.empty()) return true;
+  const VisConfig* c0 = specs[0].c;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const MfmaJobSpec& sp = specs[k];
+    const MfmaPlanned& pl = bt.plans[k];
+    hipError_t e;
+    if (pl.bc)
+      e = launch_prep_packets_mapped(sp.g->d_constraints, sp.g->d_radii, pl.bc->slot_map.as<int>(), pl.n_slots,
+                                     sp.c->step_size.x, sp.c->step_size.y, sp.u, sp.v,
+                                     const_cast<Packet*>(pl.job.packets), s);
+    else
+      e = launch_prep_packets(sp.g->d_constraints, sp.g->d_radii, sp.n, pl.n_slots, sp.c->step_size.x,
+                              sp.c->step_size.y, sp.u, sp.v, const_cast<Packet*>(pl.job.packets), s);
+    if (!hip_ok(e, "prep_packets launch")) return false;
+  }
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, bt.counters, static_cast<int>(bt.keys.size()));
+  if (!hip_ok(hipGetLastError(), "zero counters")) return false;
+
+  MfmaShared sh{};
+  sh.cx = specs[0].vp.cx;
+  sh.cy = specs[0].vp.cy;
+  sh.sx = specs[0].vp.sx;
+  sh.sy = specs[0].vp.sy;
+  const int mode = specs[0].vp.phi_mode;
+  const float sigma = (mode == VIS_PHI_REFERENCE) ? 1.0f : -1.0f;
+  sh.ref_hi = sigma > 0.0f ? 1 : 0;
   if (ev && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
-  const int form = c->form;
-  const int4* ck = ws->chunks.as<int4>();
-  float2* part = ws->partial.as<float2>();
-  if (form == VIS_FORM_MFMA_F16)
-    launch_kind<OP_F16, false>(grid, s, ws->packets.as<Packet>(), p, ck, part, gauss);
-  else if (form == VIS_FORM_MFMA_SPLIT_BF16)
-    launch_kind<OP_BF16, true>(grid, s, ws->packets.as<Packet>(), p, ck, part, gauss);
-  else
-    launch_kind<OP_F16, true>(grid, s, ws->packets.as<Packet>(), p, ck, part, gauss);
-  if (!hip_ok(hipGetLastError(), "mfma_sum launch")) return false;
+  for (size_t k = 0; k < bt.keys.size(); ++k) {
+    const int kind = std::get<0>(bt.keys[k]), basis = std::get<1>(bt.keys[k]), T = std::get<2>(bt.keys[k]);
+    sh.n_items = bt.item_begin[k + 1] - bt.item_begin[k];
+    sh.sigma_pi_log2e = kind == KIND_GAUSS ? sigma * kPi * kLog2e : 0.0f;
+    const int grid = std::max(1, std::min(sh.n_items, device_cus()));
+    const MfmaJob* kj = bt.jobs + bt.job_begin[k];
+    const MfmaItem* ki = bt.items + bt.item_begin[k];
+    int* ctr = bt.counters + k;
+    if (kind == KIND_BINNED) {
+      if (!launch_binned(grid, s, kj, ki, sh, ctr, c0->form, T, basis, mode)) return false;
+    } else if (c0->form == VIS_FORM_MFMA_F16) {
+      launch_sum_kind<OP_F16, false>(grid, s, kj, ki, sh, ctr, kind, basis, mode);
+    } else if (c0->form == VIS_FORM_MFMA_SPLIT_BF16) {
+      launch_sum_kind<OP_BF16, true>(grid, s, kj, ki, sh, ctr, kind, basis, mode);
+    } else {
+      launch_sum_kind<OP_F16, true>(grid, s, kj, ki, sh, ctr, kind, basis, mode);
+    }
+    if (!hip_ok(hipGetLastError(), "mfma launch")) return false;
+  }
   if (ev && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
-  return launch_mfma_reduce(part, nchunks, vp.cx, vp.cy, first, acc, s);
+
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
+    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, bt.plans[k].job.partial,
+                       bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
+    if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
+  }
+  return true;
 }
 
-bool launch_mfma_reduce(const double2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s) {
-  const int nfreq = 2 * cx * cy;
-  hipLaunchKernelGGL(mfma_reduce_kernel<double2>, dim3((nfreq + 255) / 256), dim3(256), 0, s, partial, ksplit, cx, cy,
-                     first ? 1 : 0, acc);
-  return hip_ok(hipGetLastError(), "mfma_reduce launch");
-}
-
-bool launch_mfma_reduce(const float2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s) {
-  const int nfreq = 2 * cx * cy;
-  hipLaunchKernelGGL(mfma_reduce_kernel<float2>, dim3((nfreq + 255) / 256), dim3(256), 0, s, partial, ksplit, cx, cy,
-                     first ? 1 : 0, acc);
-  return hip_ok(hipGetLastError(), "mfma_reduce launch");
+bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev) {
+  return mfma_prepare(specs, s, ws, &ws->batch) && mfma_enqueue(specs, ws->batch, s, ev);
 }
 
 }  // namespace mv

@@ -30,184 +30,179 @@
 
 namespace mv {
 
-constexpr int kBTileX = 128;
-constexpr int kBTileY = 64;
 constexpr int kBFlushSteps = 64;  // fold every 1024 particles
 
-struct BinnedParams {
-  int cx, cy;
-  float sx, sy;
-  int tiles_x;
-  int steps_total;
-  int steps_per_split;
-  const uint16_t* step_bin;  // bin of every 16-particle step
-  const float* nodes;        // [bin][kMaxNodes] Chebyshev nodes in h
-  const float* inv_den;      // [bin][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
-  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
-};
-
 template <int OP, bool SPLIT, int T, int BASIS, int MODE>
-__global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const Packet* __restrict__ pk, BinnedParams p,
-                                                                    double2* __restrict__ partial) {
-  __shared__ uint32_t ldsA[2][2][kBTileX * kPitch];     // [buf][hi, lo]
-  __shared__ uint32_t ldsB[2][T][2][kBTileY * kPitch];  // [buf][node][hi, lo]
+__global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob* __restrict__ jobs,
+                                                                    const MfmaItem* __restrict__ items,
+                                                                    MfmaShared sh, int* __restrict__ counter) {
+  __shared__ uint32_t ldsA[2][2][kBinnedTileX * kPitch];     // [buf][hi, lo]
+  __shared__ uint32_t ldsB[2][T][2][kBinnedTileY * kPitch];  // [buf][node][hi, lo]
+  __shared__ int next_item;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wx = wave & 3;   // 32-row block
   const int wy = wave >> 2;  // 32-column block
-  const int j0 = (blockIdx.x % p.tiles_x) * kBTileX;
-  const int y0 = (blockIdx.x / p.tiles_x) * kBTileY;
-  const int ks = blockIdx.y;
-
   const int gp = tid & (kStep - 1);
   const int gg = tid >> 4;  // 0..31
-  uint32_t ux[4], uy[2];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) ux[m] = static_cast<uint32_t>(j0 + gg + 32 * m - p.cx + 1);
-#pragma unroll
-  for (int m = 0; m < 2; ++m) uy[m] = static_cast<uint32_t>(y0 + gg + 32 * m);
+  const int two_cx = 2 * sh.cx;
 
-  auto generate = [&](int step, int buf) {
-    const Packet q = pk[static_cast<size_t>(step) * kStep + gp];
-    const int b = p.step_bin[step];
-    const float* nd = p.nodes + b * kMaxNodes;
-    const float* id = p.inv_den + b * kMaxNodes;
-    float l[T];
+  for (;;) {
+    if (tid == 0) next_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next_item;
+    __syncthreads();
+    if (it >= sh.n_items) break;
+    const MfmaItem item = items[it];
+    const MfmaJob job = jobs[item.job];
+    const int j0 = (item.tile % job.tiles_x) * kBinnedTileX;
+    const int y0 = (item.tile / job.tiles_x) * kBinnedTileY;
+
+    uint32_t ux[4], uy[2];
 #pragma unroll
-    for (int t = 0; t < T; ++t) {
-      float prod = id[t];
+    for (int m = 0; m < 4; ++m) ux[m] = static_cast<uint32_t>(j0 + gg + 32 * m - sh.cx + 1);
 #pragma unroll
-      for (int s = 0; s < T; ++s)
-        if (s != t) prod *= (q.h - nd[s]);
-      l[t] = prod;
-    }
-    const float wq = q.w * p.bininfo[b].w;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const uint32_t P = ux[m] * q.pu;
-      const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-      const float re = cos_rev(ph), im = sin_rev(ph);
-      const uint32_t hi = pack2<OP>(re, im);
-      const int o = (gg + 32 * m) * kPitch + gp;
-      ldsA[buf][0][o] = hi;
-      if (SPLIT) {
-        float hr, hm;
-        unpack2<OP>(hi, &hr, &hm);
-        ldsA[buf][1][o] = pack2<OP>(re - hr, im - hm);
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const uint32_t P = uy[m] * q.pv;
-      const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-      const float re0 = wq * cos_rev(ph), im0 = wq * sin_rev(ph);
-      const int o = (gg + 32 * m) * kPitch + gp;
+    for (int m = 0; m < 2; ++m) uy[m] = static_cast<uint32_t>(y0 + gg + 32 * m);
+
+    auto generate = [&](int step, int buf) {
+      const Packet q = job.packets[static_cast<size_t>(step) * kStep + gp];
+      const int b = job.step_bin[step];
+      const float* nd = job.nodes + b * kMaxNodes;
+      const float* id = job.inv_den + b * kMaxNodes;
+      float l[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        const float re = l[t] * re0, im = l[t] * im0;
+        float prod = id[t];
+#pragma unroll
+        for (int s = 0; s < T; ++s)
+          if (s != t) prod *= (q.h - nd[s]);
+        l[t] = prod;
+      }
+      const float wq = q.w * job.bininfo[b].w;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const uint32_t P = ux[m] * q.pu;
+        const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+        const float re = cos_rev(ph), im = sin_rev(ph);
         const uint32_t hi = pack2<OP>(re, im);
-        ldsB[buf][t][0][o] = hi;
+        const int o = (gg + 32 * m) * kPitch + gp;
+        ldsA[buf][0][o] = hi;
         if (SPLIT) {
           float hr, hm;
           unpack2<OP>(hi, &hr, &hm);
-          ldsB[buf][t][1][o] = pack2<OP>(re - hr, im - hm);
+          ldsA[buf][1][o] = pack2<OP>(re - hr, im - hm);
         }
       }
+#pragma unroll
+      for (int m = 0; m < 2; ++m) {
+        const uint32_t P = uy[m] * q.pv;
+        const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+        const float re0 = wq * cos_rev(ph), im0 = wq * sin_rev(ph);
+        const int o = (gg + 32 * m) * kPitch + gp;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const float re = l[t] * re0, im = l[t] * im0;
+          const uint32_t hi = pack2<OP>(re, im);
+          ldsB[buf][t][0][o] = hi;
+          if (SPLIT) {
+            float hr, hm;
+            unpack2<OP>(hi, &hr, &hm);
+            ldsB[buf][t][1][o] = pack2<OP>(re - hr, im - hm);
+          }
+        }
+      }
+    };
+
+    f32x16_t acc[T][2], sum[2];
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t][0] = acc[t][1] = (f32x16_t){};
+    sum[0] = sum[1] = (f32x16_t){};
+
+    auto consume = [&](int buf) {
+#pragma unroll
+      for (int kk = 0; kk < kStep / 8; ++kk) {
+        const int pofs = kk * 8 + (lane >> 5) * 4;
+        const int row = wx * 32 + (lane & 31);
+        const int col = wy * 32 + (lane & 31);
+        const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kPitch + pofs]);
+        u32x4_t alo;
+        if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kPitch + pofs]);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][0][col * kPitch + pofs]);
+          const u32x4_t bre_hi = negate_imag(bhi), bim_hi = swap_halves(bhi);
+          acc[t][0] = mfma<OP>(ahi, bre_hi, acc[t][0]);
+          acc[t][1] = mfma<OP>(ahi, bim_hi, acc[t][1]);
+          if (SPLIT) {
+            const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][1][col * kPitch + pofs]);
+            acc[t][0] = mfma<OP>(ahi, negate_imag(blo), acc[t][0]);
+            acc[t][1] = mfma<OP>(ahi, swap_halves(blo), acc[t][1]);
+            acc[t][0] = mfma<OP>(alo, bre_hi, acc[t][0]);
+            acc[t][1] = mfma<OP>(alo, bim_hi, acc[t][1]);
+          }
+        }
+      }
+    };
+
+    const int y = y0 + wy * 32 + (lane & 31);
+    const float fv = sh.sy * static_cast<float>(y);
+    float rr[16];  // |k| of the lane's 16 outputs
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+      const float fu = sh.sx * static_cast<float>(j - sh.cx + 1);
+      rr[i] = sqrtf(fu * fu + fv * fv);
     }
-  };
-
-  f32x16_t acc[T][2], sum[2];
-#pragma unroll
-  for (int t = 0; t < T; ++t) acc[t][0] = acc[t][1] = (f32x16_t){};
-  sum[0] = sum[1] = (f32x16_t){};
-
-  auto consume = [&](int buf) {
-#pragma unroll
-    for (int kk = 0; kk < kStep / 8; ++kk) {
-      const int pofs = kk * 8 + (lane >> 5) * 4;
-      const int row = wx * 32 + (lane & 31);
-      const int col = wy * 32 + (lane & 31);
-      const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kPitch + pofs]);
-      u32x4_t alo;
-      if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kPitch + pofs]);
+    // sum += W_b sum_t Phi(r h_bt) acc_t; acc = 0.
+    auto fold = [&](int b) {
+      const float* nd = job.nodes + b * kMaxNodes;
+      const float ws = job.bininfo[b].z;
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][0][col * kPitch + pofs]);
-        const u32x4_t bre_hi = negate_imag(bhi), bim_hi = swap_halves(bhi);
-        acc[t][0] = mfma<OP>(ahi, bre_hi, acc[t][0]);
-        acc[t][1] = mfma<OP>(ahi, bim_hi, acc[t][1]);
-        if (SPLIT) {
-          const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][t][1][col * kPitch + pofs]);
-          acc[t][0] = mfma<OP>(ahi, negate_imag(blo), acc[t][0]);
-          acc[t][1] = mfma<OP>(ahi, swap_halves(blo), acc[t][1]);
-          acc[t][0] = mfma<OP>(alo, bre_hi, acc[t][0]);
-          acc[t][1] = mfma<OP>(alo, bim_hi, acc[t][1]);
+        const float hn = nd[t];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+          const float f = phi<BASIS, MODE>(rr[i] * hn) * ws;
+          sum[0][i] = fmaf(f, acc[t][0][i], sum[0][i]);
+          sum[1][i] = fmaf(f, acc[t][1][i], sum[1][i]);
+          __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
         }
+        acc[t][0] = acc[t][1] = (f32x16_t){};
       }
-    }
-  };
+    };
 
-  const int y = y0 + wy * 32 + (lane & 31);
-  const float fv = p.sy * static_cast<float>(y);
-  // sum += sum_t Phi(r h_bt) acc_t; acc = 0.
-  float rr[16];  // |k| of the lane's 16 outputs
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-    const float fu = p.sx * static_cast<float>(j - p.cx + 1);
-    rr[i] = sqrtf(fu * fu + fv * fv);
-  }
-  auto fold = [&](int b) {
-    const float* nd = p.nodes + b * kMaxNodes;
-    const float ws = p.bininfo[b].z;
-#pragma unroll
-    for (int t = 0; t < T; ++t) {
-      const float hn = nd[t];
+    int cur = (item.s0 < item.s1) ? job.step_bin[item.s0] : 0;
+    int since = 0;
+    if (item.s0 < item.s1) generate(item.s0, 0);
+    __syncthreads();
+    for (int s = item.s0; s < item.s1; ++s) {
+      const int buf = (s - item.s0) & 1;
+      if (s + 1 < item.s1) generate(s + 1, buf ^ 1);
+      const int b = job.step_bin[s];
+      if (b != cur) {
+        fold(cur);
+        cur = b;
+        since = 0;
+      }
+      consume(buf);
+      if (++since == kBFlushSteps) {
+        fold(cur);
+        since = 0;
+      }
+      __syncthreads();
+    }
+    if (since) fold(cur);
+
+    if (y < sh.cy) {
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const float f = phi<BASIS, MODE>(rr[i] * hn) * ws;
-        sum[0][i] = fmaf(f, acc[t][0][i], sum[0][i]);
-        sum[1][i] = fmaf(f, acc[t][1][i], sum[1][i]);
-        __builtin_amdgcn_sched_barrier(0);  // evaluate one output at a time: keeps the fold's registers few
+        const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        if (j >= two_cx) continue;
+        job.partial[(static_cast<size_t>(item.chunk) * two_cx + j) * sh.cy + y] = make_float2(sum[0][i], sum[1][i]);
       }
-      acc[t][0] = acc[t][1] = (f32x16_t){};
     }
-  };
-
-  const int s_begin = ks * p.steps_per_split;
-  const int s_end = min(p.steps_total, s_begin + p.steps_per_split);
-  int cur = (s_begin < s_end) ? p.step_bin[s_begin] : 0;
-  int since = 0;
-  if (s_begin < s_end) generate(s_begin, 0);
-  __syncthreads();
-  for (int s = s_begin; s < s_end; ++s) {
-    const int buf = (s - s_begin) & 1;
-    if (s + 1 < s_end) generate(s + 1, buf ^ 1);
-    const int b = p.step_bin[s];
-    if (b != cur) {
-      fold(cur);
-      cur = b;
-      since = 0;
-    }
-    consume(buf);
-    if (++since == kBFlushSteps) {
-      fold(cur);
-      since = 0;
-    }
-    __syncthreads();
-  }
-  if (since) fold(cur);
-
-  if (y >= p.cy) return;
-  const int two_cx = 2 * p.cx;
-#pragma unroll
-  for (int i = 0; i < 16; ++i) {
-    const int j = j0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-    if (j >= two_cx) continue;
-    partial[(static_cast<size_t>(ks) * two_cx + j) * p.cy + y] =
-        make_double2(static_cast<double>(sum[0][i]), static_cast<double>(sum[1][i]));
   }
 }
 
@@ -331,85 +326,44 @@ const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, 
 }
 
 template <int OP, bool SPLIT, int T, int BASIS, int MODE>
-static void launch_b(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial) {
-  hipLaunchKernelGGL((mfma_binned_kernel<OP, SPLIT, T, BASIS, MODE>), grid, dim3(kThreads), 0, s, pk, p, partial);
+static void launch_b(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                     int* counter) {
+  hipLaunchKernelGGL((mfma_binned_kernel<OP, SPLIT, T, BASIS, MODE>), dim3(grid), dim3(kThreads), 0, s, jobs, items,
+                     sh, counter);
 }
 
 template <int OP, bool SPLIT, int T>
-static void launch_bt(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial, int basis,
-                      int mode) {
+static void launch_bt(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                      int* counter, int basis, int mode) {
   const bool an = mode == VIS_PHI_ANALYTIC;
   if (basis == SPH)
-    an ? launch_b<OP, SPLIT, T, SPH, VIS_PHI_ANALYTIC>(grid, s, pk, p, partial)
-       : launch_b<OP, SPLIT, T, SPH, VIS_PHI_REFERENCE>(grid, s, pk, p, partial);
+    an ? launch_b<OP, SPLIT, T, SPH, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+       : launch_b<OP, SPLIT, T, SPH, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
   else
-    an ? launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, pk, p, partial)
-       : launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, pk, p, partial);
+    an ? launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+       : launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
 }
 
 template <int OP, bool SPLIT>
-static void launch_bf(dim3 grid, hipStream_t s, const Packet* pk, const BinnedParams& p, double2* partial, int T,
-                      int basis, int mode) {
+static void launch_bf(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                      int* counter, int T, int basis, int mode) {
   switch (T) {
-    case 2: launch_bt<OP, SPLIT, 2>(grid, s, pk, p, partial, basis, mode); break;
-    case 3: launch_bt<OP, SPLIT, 3>(grid, s, pk, p, partial, basis, mode); break;
-    case 4: launch_bt<OP, SPLIT, 4>(grid, s, pk, p, partial, basis, mode); break;
-    default: launch_bt<OP, SPLIT, 5>(grid, s, pk, p, partial, basis, mode); break;
+    case 2: launch_bt<OP, SPLIT, 2>(grid, s, jobs, items, sh, counter, basis, mode); break;
+    case 3: launch_bt<OP, SPLIT, 3>(grid, s, jobs, items, sh, counter, basis, mode); break;
+    case 4: launch_bt<OP, SPLIT, 4>(grid, s, jobs, items, sh, counter, basis, mode); break;
+    default: launch_bt<OP, SPLIT, 5>(grid, s, jobs, items, sh, counter, basis, mode); break;
   }
 }
 
-bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u,
-                                double3 v, const VisConfig* c, bool first, hipStream_t s, MfmaWorkspace* ws,
-                                double2* acc, EventPair* ev) {
-  int T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 4;
-  if (T < 2 || T > kMaxNodes) {
-    set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
-    return false;
-  }
-  int nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.1);
-  if (nb > 65535) {
-    set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
-    return false;
-  }
-  const BinCache* bc = bin_cache(g, info, n, nb, T, s);
-  if (!bc) return false;
-  if (!ws->packets.reserve(sizeof(Packet) * bc->n_slots)) return false;
-  if (!hip_ok(launch_prep_packets_mapped(g.d_constraints, g.d_radii, bc->slot_map.as<int>(), bc->n_slots,
-                                         c->step_size.x, c->step_size.y, u, v, ws->packets.as<Packet>(), s),
-              "prep_packets_mapped launch"))
-    return false;
-
-  BinnedParams p;
-  p.cx = vp.cx;
-  p.cy = vp.cy;
-  p.sx = vp.sx;
-  p.sy = vp.sy;
-  p.tiles_x = (2 * vp.cx + kBTileX - 1) / kBTileX;
-  const int tiles_y = (vp.cy + kBTileY - 1) / kBTileY;
-  p.steps_total = bc->n_slots / kStep;
-  p.step_bin = bc->step_bin.as<uint16_t>();
-  p.nodes = bc->nodes.as<float>();
-  p.inv_den = bc->inv_den.as<float>();
-  p.bininfo = bc->bininfo.as<float4>();
-  const int tiles = p.tiles_x * tiles_y;
-  int ksplit = std::max(1, (1024 + tiles - 1) / tiles);
-  ksplit = std::min(ksplit, std::max(1, p.steps_total / 32));
-  p.steps_per_split = (p.steps_total + ksplit - 1) / ksplit;
-  ksplit = std::max(1, (p.steps_total + p.steps_per_split - 1) / p.steps_per_split);
-  const int nfreq = 2 * vp.cx * vp.cy;
-  if (!ws->partial.reserve(sizeof(double2) * static_cast<size_t>(ksplit) * nfreq)) return false;
-
-  const dim3 grid(tiles, ksplit);
-  if (ev && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
-  if (c->form == VIS_FORM_MFMA_F16)
-    launch_bf<OP_F16, false>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
-  else if (c->form == VIS_FORM_MFMA_SPLIT_BF16)
-    launch_bf<OP_BF16, true>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
+bool launch_binned(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                   int* counter, int form, int T, int basis, int mode) {
+  if (form == VIS_FORM_MFMA_F16)
+    launch_bf<OP_F16, false>(grid, s, jobs, items, sh, counter, T, basis, mode);
+  else if (form == VIS_FORM_MFMA_SPLIT_BF16)
+    launch_bf<OP_BF16, true>(grid, s, jobs, items, sh, counter, T, basis, mode);
   else
-    launch_bf<OP_F16, true>(grid, s, ws->packets.as<Packet>(), p, ws->partial.as<double2>(), T, vp.basis, vp.phi_mode);
-  if (!hip_ok(hipGetLastError(), "mfma_binned launch")) return false;
-  if (ev && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
-  return launch_mfma_reduce(ws->partial.as<double2>(), ksplit, vp.cx, vp.cy, first, acc, s);
+    launch_bf<OP_F16, true>(grid, s, jobs, items, sh, counter, T, basis, mode);
+  return hip_ok(hipGetLastError(), "mfma_binned launch");
 }
 
 }  // namespace mv

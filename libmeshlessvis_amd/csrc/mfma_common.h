@@ -20,22 +20,22 @@ constexpr int kTileY = 128;        // workgroup tile along y (B columns)
 constexpr int kThreads = 512;      // 8 waves
 constexpr int kProducerWaves = 4;  // waves 0-3 generate operands, waves 4-7 run the MFMAs
 constexpr int kMaxChunkSteps = 1024;  // at most 16384 particles per FP32 accumulation
+constexpr int kMaxNodes = 5;
 constexpr float kTwoPowM32 = 2.3283064365386962890625e-10f;
 constexpr float kLog2e = 1.44269504088896340736f;
 
 enum OperandType { OP_F16 = 0, OP_BF16 = 1 };
 
-struct MfmaParams {
+// Constants shared by every job of a launch.
+struct MfmaShared {
   int cx, cy;
   float sx, sy;
-  float sigma_pi_log2e;      // Gaussian: +-pi*log2(e); 0 for the uniform kind
-  float radius;              // uniform kind: Phi(r*radius)
-  int basis, phi_mode;
-  int ref_hi;                // Gaussian h_ref: 0 the bin's lower edge, 1 its upper edge
-  int tiles_x, tiles_y;
-  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|); nullptr: one bin
-  float4 bin0;               // the single bin when bininfo is nullptr
+  float sigma_pi_log2e;  // Gaussian kind: +-pi*log2(e); 0 otherwise
+  int ref_hi;            // Gaussian h_ref: 0 the bin's lower edge, 1 its upper edge
+  int n_items;
+  int pad;
 };
+
 
 template <int OP>
 __device__ __forceinline__ uint32_t pack2(float a, float b) {
@@ -80,7 +80,6 @@ __device__ __forceinline__ u32x4_t swap_halves(u32x4_t v) {  // (r, i) -> (i, r)
 // h-binning of a group (mfma_binned.hip): slot k of the per-view packet array
 // holds particle slot_map[k] (-1: zero padding); every bin starts on a
 // 16-particle step; step_bin gives each step's bin.
-constexpr int kMaxNodes = 5;
 struct BinCache {
   int n = 0, nb = 0, T = 0;
   int n_slots = 0;
@@ -103,13 +102,12 @@ const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, 
 // Bins of ratio `ratio` over [h_min, h_max], at least 1, at most 1024.
 int auto_bins(float h_min, float h_max, double ratio);
 
-// acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (the MFMA kernels' partial layout).
-bool launch_mfma_reduce(const double2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s);
-bool launch_mfma_reduce(const float2* partial, int ksplit, int cx, int cy, bool first, double2* acc, hipStream_t s);
+// Tile of the binned kernel (mfma_binned.hip).
+constexpr int kBinnedTileX = 128;
+constexpr int kBinnedTileY = 64;
 
-// SPH / Wendland with per-particle radii: h-binned, Lagrange nodes in h (mfma_binned.hip).
-bool mfma_binned_group_spectrum(const Group& g, const GroupInfo& info, int n, const ViewParams& vp, double3 u,
-                                double3 v, const VisConfig* c, bool first, hipStream_t s, MfmaWorkspace* ws,
-                                double2* acc, EventPair* ev);
+// Persistent binned kernel over a job/item table (mfma_binned.hip); false (error recorded) on failure.
+bool launch_binned(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                   int* counter, int form, int T, int basis, int mode);
 
 }  // namespace mv

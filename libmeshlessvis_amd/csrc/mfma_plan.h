@@ -1,0 +1,36 @@
+// Host plan of one MFMA job (separate so that mfma.h stays free of kernel types).
+#pragma once
+#include "mfma.h"
+
+namespace mv {
+
+struct MfmaJob {
+  const Packet* packets;     // the job's packets in slot order
+  const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|); nullptr: bin0
+  const uint16_t* step_bin;  // binned kernel: bin of every 16-particle step
+  const float* nodes;        // binned kernel: [bin][kMaxNodes] Chebyshev nodes in h
+  const float* inv_den;      // binned kernel: [bin][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+  float2* partial;           // [chunk][2cx][cy] partial spectra of the job
+  float4 bin0;               // the single bin when bininfo is nullptr
+  float radius;              // uniform-radius kind: Phi(r * radius)
+  int tiles_x;
+  int pad0, pad1;
+};
+
+// A unit of work: one output tile of one job over a range of 16-particle steps.
+struct MfmaItem {
+  int job, tile, s0, s1;  // steps [s0, s1)
+  int bin, chunk;         // the range's bin (sum kernel) and its partial slot
+  int pad0, pad1;
+};
+
+struct MfmaPlanned {
+  int kind = 0, basis = 0, T = 1;
+  const BinCache* bc = nullptr;
+  int n_slots = 0, nchunks = 0;
+  size_t pk_off = 0, part_off = 0;
+  MfmaJob job{};
+  std::vector<MfmaItem> items;
+};
+
+}  // namespace mv

@@ -79,6 +79,29 @@ void DevBuf::release() {
 }  // namespace mv
 
 // ------------------------------------------------------------------ plan ---
+// A captured batch render (vis_fourier_volume_rendering_batched), replayed while
+// its signature (configs, datasets, registrations, views, buffers) is unchanged.
+struct BatchGraph {
+  std::vector<uint64_t> key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t capture = nullptr;  // default priority
+  mv::MfmaWorkspace ws;
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    key.clear();
+  }
+  void release() {
+    drop();
+    ws.release();
+    if (capture) (void)hipStreamDestroy(capture);
+    capture = nullptr;
+  }
+};
+
 struct VisPlan {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -87,6 +110,7 @@ struct VisPlan {
   float2* tw_y = nullptr;
   mv::DevBuf acc, partial, packets;
   mv::MfmaWorkspace mfma;
+  BatchGraph batch;
   bool timing = false;
   std::vector<mv::EventPair> events;
   size_t events_used = 0;
@@ -121,6 +145,7 @@ static void plan_destroy(VisPlan* p) {
   p->partial.release();
   p->packets.release();
   p->mfma.release();
+  p->batch.release();
   for (auto& e : p->events) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -237,11 +262,9 @@ static bool group_info_for(const Group& g, GroupInfo* info) {
   return true;
 }
 
-// The spectrum of all groups into _d_freq_image slot 0.
-static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
-  VisPlan* p = c->_plan;
-  const int nfreq = static_cast<int>(spectrum_count(c));
-  if (nfreq == 0) return true;
+// Validated description of every group of a render (one MFMA job spec per group).
+static bool collect_groups(MeshlessDataset* ds, VisConfig* c, double2* acc, std::vector<MfmaJobSpec>* specs) {
+  specs->clear();
   ViewParams vp;
   vp.cx = c->_cutoff_frequency.x;
   vp.cy = c->_cutoff_frequency.y;
@@ -250,54 +273,205 @@ static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
   vp.phi_mode = c->phi_mode;
   const double3 u = make_double3(c->u_axis.x, c->u_axis.y, c->u_axis.z);
   const double3 v = make_double3(c->v_axis.x, c->v_axis.y, c->v_axis.z);
-
-  if (!p->acc.reserve(sizeof(double2) * nfreq)) return false;
   for (int gi = 0; gi < ds->number_of_groups; ++gi) {
     const Group& g = ds->groups[gi];
     if (g.number_of_terms > 0 && g.d_constraints == nullptr) {
       set_error(VIS_ERROR_NOT_REGISTERED, "group " + std::to_string(gi) + " is not registered");
       return false;
     }
-    GroupInfo info;
-    group_info_for(g, &info);
-    const int n = std::max(0, std::min(info.n, g.d_number_of_terms > 0 ? g.d_number_of_terms : info.n));
-    vp.basis = g.basis_function_id;
-    if (vp.basis < SPH || vp.basis > WENDLAND_D3_C2) {
+    MfmaJobSpec sp;
+    sp.g = &g;
+    group_info_for(g, &sp.info);
+    sp.n = std::max(0, std::min(sp.info.n, g.d_number_of_terms > 0 ? g.d_number_of_terms : sp.info.n));
+    sp.vp = vp;
+    sp.vp.basis = g.basis_function_id;
+    if (sp.vp.basis < SPH || sp.vp.basis > WENDLAND_D3_C2) {
       set_error(VIS_ERROR_INVALID_VALUE, "unknown basis_function_id");
       return false;
     }
-    if (c->form != VIS_FORM_PLAIN) {
-      if (!mfma_group_spectrum(g, info, n, vp, u, v, c, gi == 0, p->stream, &p->mfma, p->acc.as<double2>(),
-                               p->timing ? next_events(p) : nullptr))
-        return false;
-      continue;
-    }
-    const int n_padded = ((n + 255) / 256) * 256;
-    if (!p->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
-    if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
-                                    p->packets.as<Packet>(), p->stream),
-                "prep_packets launch"))
-      return false;
-    const int ntiles = n_padded / 256;
-    const int freq_tiles = ((2 * vp.cx + 63) / 64) * ((vp.cy + 31) / 32);
-    const int ksplit = choose_ksplit(ntiles, freq_tiles);
-    if (!p->partial.reserve(sizeof(double2) * static_cast<size_t>(ksplit) * nfreq)) return false;
-    EventPair* ev = p->timing ? next_events(p) : nullptr;
-    if (ev && !hip_ok(hipEventRecord(ev->first, p->stream), "hipEventRecord")) return false;
-    const bool uniform = info.uniform_radius;
-    const float radius = info.has_radii ? info.radius : 1.0f;
-    if (!hip_ok(launch_direct_sum(p->packets.as<Packet>(), n_padded, ksplit, uniform ? 1 : 0, radius, vp,
-                                  p->partial.as<double2>(), p->stream),
-                "direct_sum launch"))
-      return false;
-    if (ev && !hip_ok(hipEventRecord(ev->second, p->stream), "hipEventRecord")) return false;
-    if (!hip_ok(launch_reduce_partials(p->partial.as<double2>(), ksplit, nfreq, gi == 0 ? 1 : 0, p->acc.as<double2>(),
-                                       p->stream),
-                "reduce_partials launch"))
-      return false;
+    sp.u = u;
+    sp.v = v;
+    sp.c = c;
+    sp.acc = acc;
+    sp.first = gi == 0;
+    specs->push_back(sp);
+  }
+  return true;
+}
+
+// Plain FP32 direct sum of one group, added into acc.
+static bool plain_group(VisPlan* p, const MfmaJobSpec& sp, int nfreq) {
+  const Group& g = *sp.g;
+  const int n = sp.n;
+  const int n_padded = ((n + 255) / 256) * 256;
+  if (!p->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
+  if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, sp.c->step_size.x, sp.c->step_size.y, sp.u,
+                                  sp.v, p->packets.as<Packet>(), p->stream),
+              "prep_packets launch"))
+    return false;
+  const int ntiles = n_padded / 256;
+  const int freq_tiles = ((2 * sp.vp.cx + 63) / 64) * ((sp.vp.cy + 31) / 32);
+  const int ksplit = choose_ksplit(ntiles, freq_tiles);
+  if (!p->partial.reserve(sizeof(double2) * static_cast<size_t>(ksplit) * nfreq)) return false;
+  EventPair* ev = p->timing ? next_events(p) : nullptr;
+  if (ev && !hip_ok(hipEventRecord(ev->first, p->stream), "hipEventRecord")) return false;
+  const bool uniform = sp.info.uniform_radius;
+  const float radius = sp.info.has_radii ? sp.info.radius : 1.0f;
+  if (!hip_ok(launch_direct_sum(p->packets.as<Packet>(), n_padded, ksplit, uniform ? 1 : 0, radius, sp.vp,
+                                p->partial.as<double2>(), p->stream),
+              "direct_sum launch"))
+    return false;
+  if (ev && !hip_ok(hipEventRecord(ev->second, p->stream), "hipEventRecord")) return false;
+  return hip_ok(launch_reduce_partials(p->partial.as<double2>(), ksplit, nfreq, sp.first ? 1 : 0, sp.acc, p->stream),
+                "reduce_partials launch");
+}
+
+// The spectrum of all groups into _d_freq_image slot 0.
+static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
+  VisPlan* p = c->_plan;
+  const int nfreq = static_cast<int>(spectrum_count(c));
+  if (nfreq == 0) return true;
+  if (!p->acc.reserve(sizeof(double2) * nfreq)) return false;
+  std::vector<MfmaJobSpec> specs;
+  if (!collect_groups(ds, c, p->acc.as<double2>(), &specs)) return false;
+  if (c->form != VIS_FORM_PLAIN) {
+    if (!mfma_execute(specs, p->stream, &p->mfma, p->timing ? next_events(p) : nullptr)) return false;
+  } else {
+    for (const MfmaJobSpec& sp : specs)
+      if (!plain_group(p, sp, nfreq)) return false;
   }
   return hip_ok(launch_finalize_spectrum(p->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, p->stream),
                 "finalize launch");
+}
+
+
+// ------------------------------------------------------------ batched render ---
+// All renders of a batch go through one captured HIP graph when they share a
+// stream, an MFMA form, the Phi mode and the grid (cutoff, samples, steps):
+// per-job packet prep, one persistent MFMA launch per kernel variant, the
+// reductions, and every render's finalize + C2R.
+static bool batch_fusable(MeshlessDataset** ds, VisConfig** cf, int n) {
+  const VisConfig* c0 = cf[0];
+  if (c0->form == VIS_FORM_PLAIN) return false;
+  for (int i = 0; i < n; ++i) {
+    const VisConfig* c = cf[i];
+    if (c->_plan->stream != c0->_plan->stream || c->form != c0->form || c->phi_mode != c0->phi_mode ||
+        c->_cutoff_frequency.x != c0->_cutoff_frequency.x || c->_cutoff_frequency.y != c0->_cutoff_frequency.y ||
+        c->step_size.x != c0->step_size.x || c->step_size.y != c0->step_size.y ||
+        c->number_of_h_bins != c0->number_of_h_bins ||
+        c->number_of_interpolation_nodes != c0->number_of_interpolation_nodes || ds[i]->number_of_groups < 1)
+      return false;
+    for (int j = 0; j < i; ++j)
+      if (cf[j] == c) return false;  // one render per config (each owns its output)
+  }
+  return true;
+}
+
+static void key_put(std::vector<uint64_t>* k, const void* p) { k->push_back(reinterpret_cast<uintptr_t>(p)); }
+static void key_put(std::vector<uint64_t>* k, float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof(u));
+  k->push_back(u);
+}
+static void key_put(std::vector<uint64_t>* k, int i) { k->push_back(static_cast<uint32_t>(i)); }
+
+static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int n) {
+  std::vector<uint64_t> k;
+  key_put(&k, n);
+  for (int i = 0; i < n; ++i) {
+    const VisConfig* c = cf[i];
+    key_put(&k, c);
+    key_put(&k, ds[i]);
+    for (float f : {c->step_size.x, c->step_size.y, c->u_axis.x, c->u_axis.y, c->u_axis.z, c->v_axis.x, c->v_axis.y,
+                    c->v_axis.z})
+      key_put(&k, f);
+    for (int v : {c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y,
+                  c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes})
+      key_put(&k, v);
+    for (const void* q : {static_cast<const void*>(c->_d_freq_image), static_cast<const void*>(c->_d_image),
+                          static_cast<const void*>(c->_d_freq_image_arranged), static_cast<const void*>(c->_plan->tw_x),
+                          static_cast<const void*>(c->_plan->tw_y), static_cast<const void*>(c->_plan->acc.ptr),
+                          static_cast<const void*>(c->_plan->stream)})
+      key_put(&k, q);
+    key_put(&k, ds[i]->number_of_groups);
+    for (int g = 0; g < ds[i]->number_of_groups; ++g) {
+      const Group& gr = ds[i]->groups[g];
+      key_put(&k, gr.d_constraints);
+      key_put(&k, gr.d_radii);
+      key_put(&k, gr.number_of_terms);
+      key_put(&k, gr.d_number_of_terms);
+      key_put(&k, static_cast<int>(gr.basis_function_id));
+    }
+  }
+  return k;
+}
+
+static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGraph* bg) {
+  bg->drop();
+  hipStream_t s = cf[0]->_plan->stream;
+  std::vector<MfmaJobSpec> specs, one;
+  for (int i = 0; i < n; ++i) {
+    VisConfig* c = cf[i];
+    std::string why;
+    if (!config_valid(c, &why)) {
+      set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+      return false;
+    }
+    if (!c->_d_image) {
+      set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_batched: no _d_image (manual image not set)");
+      return false;
+    }
+    vis_config_compute_scale(c);
+    if (!c->_plan->acc.reserve(sizeof(double2) * std::max<size_t>(spectrum_count(c), 1))) return false;
+    if (!collect_groups(ds[i], c, c->_plan->acc.as<double2>(), &one)) return false;
+    specs.insert(specs.end(), one.begin(), one.end());
+  }
+  if (!mfma_prepare(specs, s, &bg->ws, &bg->ws.batch)) return false;
+  if (!bg->capture && !hip_ok(hipStreamCreateWithFlags(&bg->capture, hipStreamNonBlocking), "hipStreamCreate"))
+    return false;
+  if (!hip_ok(hipStreamBeginCapture(bg->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+    return false;
+  bool ok = mfma_enqueue(specs, bg->ws.batch, bg->capture, nullptr);
+  for (int i = 0; ok && i < n; ++i) {
+    VisConfig* c = cf[i];
+    const int nfreq = static_cast<int>(spectrum_count(c));
+    ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, bg->capture),
+                "finalize launch") &&
+         hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
+                           c->_number_of_samples.y, c->_plan->tw_x, c->_plan->tw_y, c->_d_freq_image_arranged,
+                           c->_d_image, bg->capture),
+                "c2r launch");
+  }
+  hipGraph_t graph = nullptr;
+  const hipError_t end = hipStreamEndCapture(bg->capture, &graph);
+  if (!ok) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return false;
+  }
+  if (!hip_ok(end, "hipStreamEndCapture")) return false;
+  bg->graph = graph;
+  if (!hip_ok(hipGraphInstantiate(&bg->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate")) {
+    bg->drop();
+    return false;
+  }
+  return true;
+}
+
+static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
+  VisPlan* p0 = cf[0]->_plan;
+  BatchGraph& bg = p0->batch;
+  std::vector<uint64_t> key = batch_key(ds, cf, n);
+  if (!bg.exec || key != bg.key) {
+    if (!capture_batch(ds, cf, n, &bg)) {
+      bg.drop();
+      return;
+    }
+    bg.key = key;
+  }
+  EventPair* ev = p0->timing ? next_events(p0) : nullptr;
+  if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
+  if (!hip_ok(hipGraphLaunch(bg.exec, p0->stream), "hipGraphLaunch")) return;
+  if (ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
 }
 
 }  // namespace mv
@@ -585,10 +759,21 @@ void vis_fourier_volume_rendering_batched(MeshlessDataset** datasets, VisConfig*
     set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_batched: bad arguments");
     return;
   }
+  if (n == 0) return;
   for (int i = 0; i < n; ++i) {
-    vis_fourier_volume_rendering(datasets[i], configs[i]);
-    if (vis_get_last_error_code() != VIS_SUCCESS) return;
+    if (!datasets[i] || !configs[i]) {
+      set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_batched: null dataset or config");
+      return;
+    }
   }
+  if (!batch_fusable(datasets, configs, n)) {
+    for (int i = 0; i < n; ++i) {
+      vis_fourier_volume_rendering(datasets[i], configs[i]);
+      if (vis_get_last_error_code() != VIS_SUCCESS) return;
+    }
+    return;
+  }
+  render_batch_graph(datasets, configs, n);
 }
 
 void vis_set_kernel_timing(VisConfig* c, int enable) {
