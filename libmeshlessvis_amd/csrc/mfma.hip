@@ -293,7 +293,7 @@ int mfma_kind(const GroupInfo& info, int basis) {
 
 // Bins, packet slots and work items of one job (host side; builds the group's
 // bins on first use).
-bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, MfmaPlanned* out) {
+bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, MfmaPlanned* out) {
   const VisConfig* c = spec.c;
   const GroupInfo& info = spec.info;
   out->kind = mfma_kind(info, spec.vp.basis);
@@ -347,8 +347,9 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, MfmaPlanned* out) {
   const long long total_steps = bin_step.back();
   const int cap = binned ? 4096 : kMaxChunkSteps;
   const int floor_steps = binned ? 128 : 32;
-  const int chunk_steps =
-      static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, total_steps * tiles / 2048)));
+  out->tile_steps = total_steps * tiles;
+  if (target_steps <= 0) target_steps = total_steps * tiles / 2048;  // one render: ~2048 items
+  const int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
   int chunk = 0;
   auto add_range = [&](int bin, int s0, int s1) {
     for (int t = 0; t < tiles; ++t) {
@@ -382,9 +383,17 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   bt->job_begin.clear();
   bt->item_begin.clear();
   if (specs.empty()) return true;
+  // Items per batch: about 16 per CU over the whole batch, so a batch of many small
+  // renders is not split more finely than it needs to be.
+  long long work = 0;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    if (!mfma_plan(specs[k], s, 0, &bt->plans[k])) return false;
+    work += bt->plans[k].tile_steps;
+  }
+  const long long target = std::max<long long>(1, work / (16LL * device_cus()));
   size_t slots = 0, parts = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
-    if (!mfma_plan(specs[k], s, &bt->plans[k])) return false;
+    if (specs.size() > 1 && !mfma_plan(specs[k], s, target, &bt->plans[k])) return false;
     bt->plans[k].pk_off = slots;
     bt->plans[k].part_off = parts;
     slots += bt->plans[k].n_slots;

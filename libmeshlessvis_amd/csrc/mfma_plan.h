@@ -28,6 +28,7 @@ struct MfmaPlanned {
   int kind = 0, basis = 0, T = 1;
   const BinCache* bc = nullptr;
   int n_slots = 0, nchunks = 0;
+  long long tile_steps = 0;  // steps x tiles: the job's work
   size_t pk_off = 0, part_off = 0;
   MfmaJob job{};
   std::vector<MfmaItem> items;
