@@ -273,6 +273,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--autotune", type=float, default=0.0, metavar="TOL",
+                    help="before timing, let vis_autotune pick nodes/bin ratio within TOL of the plain kernel")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -303,6 +305,18 @@ def main() -> None:
     cfgs = [host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
                         phi_mode=wl.phi_mode) for vw in wl.views]
     cfgs[0].register(ds)
+    tuned = None
+    if args.autotune > 0 and form != "plain":
+        t = time.perf_counter()
+        tuned_ms = cfgs[0].autotune(ds, args.autotune)
+        cs = cfgs[0].cs
+        tuned = {"tolerance": args.autotune, "nodes": cs.number_of_interpolation_nodes,
+                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "spectrum_ms": tuned_ms}
+        for c in cfgs[1:]:
+            c.cs.number_of_h_bins = 0
+            c.cs.number_of_interpolation_nodes = cs.number_of_interpolation_nodes
+            c.cs.h_bin_ratio = cs.h_bin_ratio
+        log(f"autotune: {tuned} ({time.perf_counter() - t:.1f} s)")
     log(f"registered; timing form={form}")
     elapsed, launches, kernel_ms = time_renders(cfgs, ds, args.steps, args.warmup)
     views = args.steps * len(wl.views)
@@ -386,6 +400,7 @@ def main() -> None:
             "basis": wl.basis,
             "phi_mode": wl.phi_mode,
             "form": form,
+            "autotune": tuned,
         },
         "views_per_s": views / elapsed,
         "rel_l2_vs_fp64": rel,

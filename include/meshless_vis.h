@@ -103,6 +103,8 @@ typedef struct
 	int phi_mode;                       /* enum VisPhiMode, default VIS_PHI_REFERENCE  */
 	int number_of_h_bins;               /* MFMA form, SPH/Wendland radii: 0 = auto     */
 	int number_of_interpolation_nodes;  /* MFMA form, SPH/Wendland radii: nodes per bin, 0 = auto */
+	float h_bin_ratio;                  /* MFMA form: ratio h_hi/h_lo of the radius bins, 0 = auto
+	                                       (used when number_of_h_bins is 0)                         */
 } VisConfig;
 
 /* meshless_vis.h:81 / meshless_vis.cu:129-176.  Returns 0 (and sets the last error) on failure. */
@@ -145,9 +147,22 @@ void vis_synchronize(VisConfig* vis_config);
 /* Synchronising D2H copy of the 2*cx*cy complex spectrum (slot 0, layout [y][x0]). */
 void vis_copy_spectrum_to_host(VisConfig* vis_config, float2* h_spectrum);
 
-/* Render many (dataset, config) pairs; all configs must share one stream.
-   Equivalent to calling vis_fourier_volume_rendering on each pair in order. */
+/* Render many (dataset, config) pairs; same results as calling
+   vis_fourier_volume_rendering on each pair.  When every config has its own
+   entry and they share a stream, an MFMA form, the Phi mode, cutoff, samples,
+   steps and bin settings, the whole batch runs as one captured HIP graph (one
+   persistent MFMA launch per kernel variant), replayed while the batch is
+   unchanged; otherwise the renders run one after the other. */
 void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, VisConfig** vis_configs, int number_of_renders);
+
+/* Picks number_of_interpolation_nodes and h_bin_ratio for the MFMA form on this
+   (registered) dataset: every candidate is timed on the config's stream and its
+   spectrum compared with the plain FP32 kernel's; the fastest candidate whose
+   relative L2 difference is at most `tolerance` is kept (the most accurate one
+   when none is), with number_of_h_bins set to 0 (bins from the ratio).  Leaves
+   the config's other fields as they were and overwrites _d_freq_image.  Returns
+   the kept candidate's spectrum time in milliseconds, or a negative value on error. */
+double vis_autotune(MeshlessDataset* meshless_dataset, VisConfig* vis_config, double tolerance);
 
 /* Kernel timing with HIP events on the config's stream around the direct-sum
    kernel (the dominant kernel of a render).  Off by default. */

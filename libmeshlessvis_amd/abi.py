@@ -86,6 +86,7 @@ class VisConfig(C.Structure):
         ("phi_mode", C.c_int),
         ("number_of_h_bins", C.c_int),
         ("number_of_interpolation_nodes", C.c_int),
+        ("h_bin_ratio", C.c_float),
     ]
 
 
@@ -127,6 +128,7 @@ PROTOTYPES = {
     "vis_synchronize": (None, [VisConfigP]),
     "vis_copy_spectrum_to_host": (None, [VisConfigP, C.POINTER(float2)]),
     "vis_fourier_volume_rendering_batched": (None, [C.POINTER(DatasetP), C.POINTER(VisConfigP), C.c_int]),
+    "vis_autotune": (C.c_double, [DatasetP, VisConfigP, C.c_double]),
     "vis_set_kernel_timing": (None, [VisConfigP, C.c_int]),
     "vis_get_kernel_timing": (C.c_int, [VisConfigP, C.POINTER(C.c_double)]),
     "vis_reset_kernel_timing": (None, [VisConfigP]),

@@ -62,6 +62,9 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
 // prepare + enqueue.
 bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
 
+// Which kernel variant a group takes (MfmaKind).
+int mfma_kind(const GroupInfo& info, int basis);
+
 // Drops what the MFMA forms cached about a group (called on unregister).
 void mfma_forget_group(const void* key);
 

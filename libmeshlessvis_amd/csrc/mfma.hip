@@ -312,7 +312,8 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   } else {
     int nb, T = 1;
     if (out->kind == KIND_GAUSS) {
-      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, 1.25);
+      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 1.25;
+      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
     } else {
       T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 3;
       if (T < 2 || T > kMaxNodes) {
@@ -320,7 +321,8 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
         return false;
       }
       // Defaults: 3 Chebyshev nodes per bin of ratio 1.04 (interpolation error < 3e-6 of Phi(0)).
-      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, T == 3 ? 1.04 : 1.1);
+      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 3 ? 1.04 : 1.1);
+      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
     }
     if (nb > 65535) {
       set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");

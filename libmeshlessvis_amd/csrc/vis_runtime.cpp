@@ -226,6 +226,7 @@ static bool config_valid(const VisConfig* c, std::string* why) {
   if (c->_number_of_samples.x > 4096 || c->_number_of_samples.y > 4096) { *why = "number_of_samples above 4096"; return false; }
   if (c->form < VIS_FORM_PLAIN || c->form > VIS_FORM_MFMA_SPLIT_F16) { *why = "unknown form"; return false; }
   if (c->phi_mode != VIS_PHI_REFERENCE && c->phi_mode != VIS_PHI_ANALYTIC) { *why = "unknown phi_mode"; return false; }
+  if (!(c->h_bin_ratio == 0.0f || c->h_bin_ratio > 1.0f)) { *why = "h_bin_ratio must be 0 (auto) or above 1"; return false; }
   return true;
 }
 
@@ -359,7 +360,8 @@ static bool batch_fusable(MeshlessDataset** ds, VisConfig** cf, int n) {
         c->_cutoff_frequency.x != c0->_cutoff_frequency.x || c->_cutoff_frequency.y != c0->_cutoff_frequency.y ||
         c->step_size.x != c0->step_size.x || c->step_size.y != c0->step_size.y ||
         c->number_of_h_bins != c0->number_of_h_bins ||
-        c->number_of_interpolation_nodes != c0->number_of_interpolation_nodes || ds[i]->number_of_groups < 1)
+        c->number_of_interpolation_nodes != c0->number_of_interpolation_nodes ||
+        c->h_bin_ratio != c0->h_bin_ratio || ds[i]->number_of_groups < 1)
       return false;
     for (int j = 0; j < i; ++j)
       if (cf[j] == c) return false;  // one render per config (each owns its output)
@@ -383,7 +385,7 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
     key_put(&k, c);
     key_put(&k, ds[i]);
     for (float f : {c->step_size.x, c->step_size.y, c->u_axis.x, c->u_axis.y, c->u_axis.z, c->v_axis.x, c->v_axis.y,
-                    c->v_axis.z})
+                    c->v_axis.z, c->h_bin_ratio})
       key_put(&k, f);
     for (int v : {c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y,
                   c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes})
@@ -774,6 +776,107 @@ void vis_fourier_volume_rendering_batched(MeshlessDataset** datasets, VisConfig*
     return;
   }
   render_batch_graph(datasets, configs, n);
+}
+
+double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
+  clear_error();
+  if (!ds || !c || !(tolerance > 0.0)) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_autotune: null argument or non-positive tolerance");
+    return -1.0;
+  }
+  if (c->form == VIS_FORM_PLAIN) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "vis_autotune: the config's form is not an MFMA form");
+    return -1.0;
+  }
+  std::string why;
+  if (!config_valid(c, &why)) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+    return -1.0;
+  }
+  VisPlan* p = c->_plan;
+  const size_t nfreq = spectrum_count(c);
+  if (!p->acc.reserve(sizeof(double2) * std::max<size_t>(nfreq, 1))) return -1.0;
+  std::vector<MfmaJobSpec> specs;
+  if (!collect_groups(ds, c, p->acc.as<double2>(), &specs)) return -1.0;
+  bool binned = false, gauss = false;
+  for (const MfmaJobSpec& sp : specs) {
+    const int k = mfma_kind(sp.info, sp.vp.basis);
+    binned |= k == KIND_BINNED;
+    gauss |= k == KIND_GAUSS;
+  }
+  vis_config_compute_scale(c);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!hip_ok(hipEventCreate(&e0), "hipEventCreate") || !hip_ok(hipEventCreate(&e1), "hipEventCreate")) return -1.0;
+  auto timed = [&](double* ms) {
+    if (!spectrum(ds, c)) return false;  // warm: bins, buffers
+    if (!hip_ok(hipEventRecord(e0, p->stream), "hipEventRecord")) return false;
+    for (int r = 0; r < 2; ++r)
+      if (!spectrum(ds, c)) return false;
+    if (!hip_ok(hipEventRecord(e1, p->stream), "hipEventRecord") || !hip_ok(hipEventSynchronize(e1), "sync"))
+      return false;
+    float t = 0.0f;
+    if (!hip_ok(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime")) return false;
+    *ms = t / 2.0;
+    return true;
+  };
+  auto fetch = [&](std::vector<float2>* out) {
+    out->resize(nfreq);
+    return nfreq == 0 || (hip_ok(hipMemcpyAsync(out->data(), c->_d_freq_image, sizeof(float2) * nfreq,
+                                                hipMemcpyDeviceToHost, p->stream),
+                                 "hipMemcpy(spectrum)") &&
+                          hip_ok(hipStreamSynchronize(p->stream), "synchronize"));
+  };
+  const int form = c->form, nodes0 = c->number_of_interpolation_nodes, bins0 = c->number_of_h_bins;
+  const float ratio0 = c->h_bin_ratio;
+  double best_ms = -1.0, best_err = 0.0;
+  int best_nodes = nodes0;
+  float best_ratio = ratio0;
+  bool best_ok = false;
+  std::vector<float2> ref, got;
+  bool ok = true;
+  c->form = VIS_FORM_PLAIN;  // the reference spectrum: the plain FP32 kernel
+  ok = spectrum(ds, c) && fetch(&ref);
+  c->form = form;
+  double ref_norm = 0.0;
+  for (const float2& z : ref) ref_norm += static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
+  std::vector<std::pair<int, float>> cand;
+  if (binned)
+    cand = {{2, 1.01f}, {2, 1.02f}, {3, 1.03f}, {3, 1.04f}, {3, 1.06f}, {4, 1.1f}, {4, 1.15f}, {5, 1.2f}, {5, 1.3f}};
+  else if (gauss)
+    cand = {{nodes0, 1.25f}, {nodes0, 1.5f}, {nodes0, 2.0f}, {nodes0, 4.0f}};
+  else
+    cand = {{nodes0, ratio0}};
+  c->number_of_h_bins = 0;
+  for (size_t k = 0; ok && k < cand.size(); ++k) {
+    c->number_of_interpolation_nodes = cand[k].first;
+    c->h_bin_ratio = cand[k].second;
+    double ms = 0.0;
+    if (!(ok = timed(&ms) && fetch(&got))) break;
+    double d = 0.0;
+    for (size_t i = 0; i < nfreq; ++i) {
+      const double dx = static_cast<double>(got[i].x) - ref[i].x, dy = static_cast<double>(got[i].y) - ref[i].y;
+      d += dx * dx + dy * dy;
+    }
+    const double err = ref_norm > 0.0 ? std::sqrt(d / ref_norm) : std::sqrt(d);
+    const bool meets = err <= tolerance;
+    if (best_ms < 0.0 || (meets && (!best_ok || ms < best_ms)) || (!meets && !best_ok && err < best_err)) {
+      best_ms = ms;
+      best_err = err;
+      best_ok = meets;
+      best_nodes = cand[k].first;
+      best_ratio = cand[k].second;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->number_of_h_bins = bins0;
+  c->number_of_interpolation_nodes = nodes0;
+  c->h_bin_ratio = ratio0;
+  if (!ok) return -1.0;
+  c->number_of_h_bins = 0;
+  c->number_of_interpolation_nodes = best_nodes;
+  c->h_bin_ratio = best_ratio;
+  return best_ms;
 }
 
 void vis_set_kernel_timing(VisConfig* c, int enable) {

@@ -87,6 +87,7 @@ class Config:
         phi_mode="reference",
         h_bins=0,
         nodes=0,
+        h_bin_ratio=0.0,
     ):
         L = lib()
         L.vis_clear_last_error()
@@ -108,6 +109,7 @@ class Config:
         self.phi_mode = phi_mode
         self.cs.number_of_h_bins = int(h_bins)
         self.cs.number_of_interpolation_nodes = int(nodes)
+        self.cs.h_bin_ratio = float(h_bin_ratio)
 
     # -- fields ------------------------------------------------------------
     @property
@@ -163,6 +165,13 @@ class Config:
     def fourier_transform(self, ds: Dataset) -> None:
         lib().fourier_transform(ds.ptr, self.p)
         check("fourier_transform")
+
+    def autotune(self, ds: Dataset, tolerance: float = 1e-5) -> float:
+        """vis_autotune: choose nodes per bin and bin ratio for the MFMA form on ``ds``
+        (registered); returns the chosen setting's spectrum time in ms."""
+        ms = lib().vis_autotune(ds.ptr, self.p, float(tolerance))
+        check("vis_autotune")
+        return ms
 
     def synchronize(self) -> None:
         lib().vis_synchronize(self.p)

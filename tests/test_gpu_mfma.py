@@ -134,3 +134,42 @@ def test_weights_spanning_decades(basis):
     e = oracle.rel_l2(F, F64)
     print(f"decade weights {basis}: rel-L2 {e:.3e}")
     assert e <= 1e-5
+
+
+@pytest.mark.parametrize("basis", ["WENDLAND_D3_C2", "GAUSSIAN"])
+def test_autotune_meets_tolerance(basis):
+    """vis_autotune picks (nodes, h_bin_ratio) whose spectrum is within the tolerance of the plain
+    kernel's; the chosen setting then renders within that tolerance of the FP64 oracle too."""
+    rng = np.random.default_rng(13)
+    n = 6000
+    radii = np.exp(rng.uniform(np.log(0.004), np.log(0.12), n)).astype(np.float32)
+    groups = cloud(rng, n, basis, radii)
+    cutoff, samples, step = (64, 48), (128, 128), (0.5, 0.5)
+    ds = host.Dataset(groups)
+    F64 = oracle.spectrum64(groups, cutoff, step, U, V, "analytic")
+    with host.Config(cutoff=cutoff, samples=samples, step=step, u=U, v=V, form="mfma_split_f16",
+                     phi_mode="analytic", h_bins=40) as cfg:
+        cfg.register(ds)
+        ms = cfg.autotune(ds, 1e-5)
+        assert ms > 0.0
+        assert cfg.form == "mfma_split_f16" and cfg.cs.number_of_h_bins == 0
+        assert cfg.cs.h_bin_ratio > 1.0
+        cfg.render(ds)
+        e = oracle.rel_l2(cfg.spectrum(), F64)
+        print(f"autotune {basis}: T={cfg.cs.number_of_interpolation_nodes} ratio={cfg.cs.h_bin_ratio:.3f} "
+              f"{ms:.3f} ms rel-L2 {e:.3e}")
+        assert e <= 1.2e-5
+        # An unreachable tolerance keeps the most accurate candidate, still a valid setting.
+        assert cfg.autotune(ds, 1e-12) > 0.0
+        cfg.render(ds)
+        assert oracle.rel_l2(cfg.spectrum(), F64) <= 1.2e-5
+        cfg.form = "plain"
+        with pytest.raises(host.abi.VisError) as ei:
+            cfg.autotune(ds, 1e-5)
+        assert ei.value.name == "VIS_ERROR_INVALID_CONFIG"
+        cfg.form = "mfma_split_f16"
+        cfg.cs.h_bin_ratio = 0.5
+        with pytest.raises(host.abi.VisError) as ei:
+            cfg.render(ds)
+        assert ei.value.name == "VIS_ERROR_INVALID_CONFIG"
+        cfg.unregister(ds)
