@@ -77,32 +77,38 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
       const int ptid = role_idx * 64 + lane;
       const int gp = ptid & (kStep - 1);  // particle of the step
       const int gg = ptid >> 4;           // 0..15: rows / columns gg + 16m
-      uint32_t ux[8], uy[8];
+      // Row gg + 16m of the tile: phase = P0 + m * 16 * (phase step), exact in
+      // 32-bit modular arithmetic.
+      const uint32_t x_first = static_cast<uint32_t>(j0 + gg - sh.cx + 1);
+      const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
       float ca[8], cb[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const int x = j0 + gg + 16 * m - sh.cx + 1;
-        const int y = y0 + gg + 16 * m;
-        ux[m] = static_cast<uint32_t>(x);
-        uy[m] = static_cast<uint32_t>(y);
-        const float fu = sh.sx * static_cast<float>(x);
-        const float fv = sh.sy * static_cast<float>(y);
+        const float fu = sh.sx * static_cast<float>(static_cast<int>(x_first) + 16 * m);
+        const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
         ca[m] = sh.sigma_pi_log2e * fu * fu;
         cb[m] = sh.sigma_pi_log2e * fv * fv;
       }
-      auto generate = [&](int step, int buf) {
-        const Packet q = job.packets[static_cast<size_t>(step) * kStep + gp];
-        const float dh = q.h * q.h - hh_ref;
+      const Packet* pk = job.packets + gp;
+      auto generate = [&](const Packet& q, int buf) {
+        // h_ref is the bin edge that keeps the exponents <= 0 (|alpha|, |beta| <= 1).
+        float dh = q.h * q.h - hh_ref;
+        dh = sh.ref_hi ? fminf(dh, 0.0f) : fmaxf(dh, 0.0f);
         const float wq = q.w * bi.w;
+        uint32_t PA = x_first * q.pu + kPhaseRound, PB = y_first * q.pv + kPhaseRound;
+        const uint32_t dA = q.pu << 4, dB = q.pv << 4;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
           const int o = (gg + 16 * m) * kPitch + gp;
           {  // A(x, i)
-            const uint32_t P = ux[m] * q.pu;
-            const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-            float amp = 1.0f;
-            if (GAUSS) amp = __builtin_amdgcn_exp2f(fminf(ca[m] * dh, 0.0f));
-            const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
+            const float ph = phase_rev(PA);
+            PA += dA;
+            float re = cos_rev(ph), im = sin_rev(ph);
+            if (GAUSS) {
+              const float amp = __builtin_amdgcn_exp2f(ca[m] * dh);
+              re *= amp;
+              im *= amp;
+            }
             const uint32_t hi = pack2<OP>(re, im);
             lds[buf][0][o] = hi;
             if (SPLIT) {
@@ -112,10 +118,10 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
             }
           }
           {  // B(i, y)
-            const uint32_t P = uy[m] * q.pv;
-            const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+            const float ph = phase_rev(PB);
+            PB += dB;
             float amp = wq;
-            if (GAUSS) amp *= __builtin_amdgcn_exp2f(fminf(cb[m] * dh, 0.0f));
+            if (GAUSS) amp *= __builtin_amdgcn_exp2f(cb[m] * dh);
             const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
             const uint32_t hi = pack2<OP>(re, im);
             lds[buf][2][o] = hi;
@@ -127,10 +133,21 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
           }
         }
       };
-      if (item.s0 < item.s1) generate(item.s0, 0);
+      // Packets are loaded one step ahead of their use, so the load's latency
+      // hides behind a whole step of operand generation.
+      Packet qn{};
+      if (item.s0 < item.s1) {
+        const Packet q0 = load_packet(pk + static_cast<size_t>(item.s0) * kStep);
+        if (item.s0 + 1 < item.s1) qn = load_packet(pk + static_cast<size_t>(item.s0 + 1) * kStep);
+        generate(q0, 0);
+      }
       __syncthreads();
       for (int s = item.s0; s < item.s1; ++s) {
-        if (s + 1 < item.s1) generate(s + 1, (s - item.s0 + 1) & 1);
+        if (s + 1 < item.s1) {
+          const Packet q = qn;
+          if (s + 2 < item.s1) qn = load_packet(pk + static_cast<size_t>(s + 2) * kStep);
+          generate(q, (s - item.s0 + 1) & 1);
+        }
         __syncthreads();
       }
       continue;
@@ -336,7 +353,7 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
     job.bininfo = bc->bininfo.as<float4>();
     job.step_bin = bc->step_bin.as<uint16_t>();
     job.nodes = bc->nodes.as<float>();
-    job.inv_den = bc->inv_den.as<float>();
+    job.lag = bc->lag.as<float4>();
     bin_step = bc->bin_step;
   }
   // Work items.  Sum kernel: bin-aligned step ranges of at most kMaxChunkSteps; binned

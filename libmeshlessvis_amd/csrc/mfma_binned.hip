@@ -8,6 +8,8 @@
 //     S(x, y)    ~= sum_b sum_t Phi(r h_bt) * sum_{i in b} A(x, i) B_t(i, y)
 //     A(x, i) = exp(+2 pi i x a_i),  B_t(i, y) = (w_i / W_b) l_bt(h_i) exp(+2 pi i y b_i)
 //
+// (l_bt(h_i) / W_b is computed once per binning, in FP64 on the host, and kept
+// per particle slot)
 // with W_b the largest |w| of bin b, folded back in with the node factors
 // (weights correlate with h, so per-bin scaling keeps the FP16 operands away
 // from the subnormal range);
@@ -60,31 +62,39 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
     const int j0 = (item.tile % job.tiles_x) * kBinnedTileX;
     const int y0 = (item.tile / job.tiles_x) * kBinnedTileY;
 
-    uint32_t ux[4], uy[2];
+    // Rows gg + 32m, columns gg + 32m: phase = P0 + m * 32 * (phase step), exact
+    // in 32-bit modular arithmetic.
+    const uint32_t x_first = static_cast<uint32_t>(j0 + gg - sh.cx + 1);
+    const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
+    constexpr int LW = lag_words(T);
+    struct Ahead {
+      Packet q;
+      float4 l[LW];
+    };
+    auto fetch = [&](int step, Ahead* a) {
+      const size_t slot = static_cast<size_t>(step) * kStep + gp;
+      a->q = load_packet(job.packets + slot);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) ux[m] = static_cast<uint32_t>(j0 + gg + 32 * m - sh.cx + 1);
-#pragma unroll
-    for (int m = 0; m < 2; ++m) uy[m] = static_cast<uint32_t>(y0 + gg + 32 * m);
+      for (int k = 0; k < LW; ++k) a->l[k] = load_float4(job.lag + slot * LW + k);
+    };
+    auto step_bin = [&](int step) -> int {
+      return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + step);
+    };
 
-    auto generate = [&](int step, int buf) {
-      const Packet q = job.packets[static_cast<size_t>(step) * kStep + gp];
-      const int b = job.step_bin[step];
-      const float* nd = job.nodes + b * kMaxNodes;
-      const float* id = job.inv_den + b * kMaxNodes;
+    auto generate = [&](const Ahead& a, int buf) {
+      const Packet& q = a.q;
       float l[T];
 #pragma unroll
       for (int t = 0; t < T; ++t) {
-        float prod = id[t];
-#pragma unroll
-        for (int s = 0; s < T; ++s)
-          if (s != t) prod *= (q.h - nd[s]);
-        l[t] = prod;
+        const float4 v = a.l[t >> 2];
+        l[t] = (t & 3) == 0 ? v.x : (t & 3) == 1 ? v.y : (t & 3) == 2 ? v.z : v.w;
       }
-      const float wq = q.w * job.bininfo[b].w;
+      uint32_t PA = x_first * q.pu + kPhaseRound;
+      const uint32_t dA = q.pu << 5;
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        const uint32_t P = ux[m] * q.pu;
-        const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
+        const float ph = phase_rev(PA);
+        PA += dA;
         const float re = cos_rev(ph), im = sin_rev(ph);
         const uint32_t hi = pack2<OP>(re, im);
         const int o = (gg + 32 * m) * kPitch + gp;
@@ -95,11 +105,13 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
           ldsA[buf][1][o] = pack2<OP>(re - hr, im - hm);
         }
       }
+      uint32_t PB = y_first * q.pv + kPhaseRound;
+      const uint32_t dB = q.pv << 5;
 #pragma unroll
       for (int m = 0; m < 2; ++m) {
-        const uint32_t P = uy[m] * q.pv;
-        const float ph = static_cast<float>(static_cast<int32_t>(P)) * kTwoPowM32;
-        const float re0 = wq * cos_rev(ph), im0 = wq * sin_rev(ph);
+        const float ph = phase_rev(PB);
+        PB += dB;
+        const float re0 = q.w * cos_rev(ph), im0 = q.w * sin_rev(ph);
         const int o = (gg + 32 * m) * kPitch + gp;
 #pragma unroll
         for (int t = 0; t < T; ++t) {
@@ -173,14 +185,34 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
       }
     };
 
-    int cur = (item.s0 < item.s1) ? job.step_bin[item.s0] : 0;
-    int since = 0;
-    if (item.s0 < item.s1) generate(item.s0, 0);
+    // Packets, Lagrange weights and step bins are loaded one step ahead of their
+    // use, so their latency hides behind a whole step.
+    Ahead an{};
+    int cur = 0, bn = 0;  // bin of the first step; bin of the step after the current one
+    if (item.s0 < item.s1) {
+      Ahead a0;
+      fetch(item.s0, &a0);
+      cur = step_bin(item.s0);
+      if (item.s0 + 1 < item.s1) {
+        fetch(item.s0 + 1, &an);
+        bn = step_bin(item.s0 + 1);
+      }
+      generate(a0, 0);
+    }
     __syncthreads();
+    int since = 0;
+    int b = cur;  // bin of step s
     for (int s = item.s0; s < item.s1; ++s) {
       const int buf = (s - item.s0) & 1;
-      if (s + 1 < item.s1) generate(s + 1, buf ^ 1);
-      const int b = job.step_bin[s];
+      const int b_next = bn;
+      if (s + 1 < item.s1) {
+        const Ahead a = an;
+        if (s + 2 < item.s1) {
+          fetch(s + 2, &an);
+          bn = step_bin(s + 2);
+        }
+        generate(a, buf ^ 1);
+      }
       if (b != cur) {
         fold(cur);
         cur = b;
@@ -191,6 +223,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
         fold(cur);
         since = 0;
       }
+      b = b_next;
       __syncthreads();
     }
     if (since) fold(cur);
@@ -280,17 +313,34 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   std::vector<uint16_t> sbin(steps, 0);
   for (int b = 0; b < nb; ++b)
     for (int k = start[b] / kStep; k < start[b + 1] / kStep; ++k) sbin[k] = static_cast<uint16_t>(b);
-  std::vector<float> nodes(static_cast<size_t>(nb) * kMaxNodes, 0.0f), inv(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
+  std::vector<float> nodes(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
+  std::vector<double> ndd(static_cast<size_t>(nb) * kMaxNodes, 0.0), inv(static_cast<size_t>(nb) * kMaxNodes, 0.0);
   for (int b = 0; b < nb; ++b) {
     const double c = 0.5 * (edge[b] + edge[b + 1]), d = 0.5 * (edge[b + 1] - edge[b]);
-    double nd[kMaxNodes];
+    double* nd = &ndd[static_cast<size_t>(b) * kMaxNodes];
     for (int t = 0; t < T; ++t) nd[t] = c + d * std::cos((2.0 * t + 1.0) * M_PI / (2.0 * T));
     for (int t = 0; t < T; ++t) {
       double den = 1.0;
       for (int q = 0; q < T; ++q)
         if (q != t) den *= (nd[t] - nd[q]);
       nodes[b * kMaxNodes + t] = static_cast<float>(nd[t]);
-      inv[b * kMaxNodes + t] = static_cast<float>(1.0 / den);
+      inv[b * kMaxNodes + t] = 1.0 / den;
+    }
+  }
+  // Lagrange weights of every particle at its bin's nodes, scaled by 1 / max |w|
+  // of the bin (view-independent, so computed once here).
+  const int lw = 4 * lag_words(T);
+  std::vector<float> lag(static_cast<size_t>(n_slots) * lw, 0.0f);
+  for (int k = 0; k < n_slots; ++k) {
+    const int i = map[k];
+    if (i < 0) continue;
+    const int b = bin_of(h[i]);
+    const double* nd = &ndd[static_cast<size_t>(b) * kMaxNodes];
+    for (int t = 0; t < T; ++t) {
+      double l = inv[static_cast<size_t>(b) * kMaxNodes + t] * binfo[b].w;
+      for (int q = 0; q < T; ++q)
+        if (q != t) l *= (static_cast<double>(h[i]) - nd[q]);
+      lag[static_cast<size_t>(k) * lw + t] = static_cast<float>(l);
     }
   }
   bc->n = n;
@@ -300,13 +350,13 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   bc->bin_step.resize(nb + 1);
   for (int b = 0; b <= nb; ++b) bc->bin_step[b] = start[b] / kStep;
   if (!bc->slot_map.reserve(sizeof(int) * n_slots) || !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
-      !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->inv_den.reserve(sizeof(float) * inv.size()) ||
+      !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->lag.reserve(sizeof(float) * lag.size()) ||
       !bc->bininfo.reserve(sizeof(float4) * nb))
     return false;
   return hip_ok(hipMemcpy(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots, hipMemcpyHostToDevice), "upload") &&
          hip_ok(hipMemcpy(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps, hipMemcpyHostToDevice), "upload") &&
          hip_ok(hipMemcpy(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size(), hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->inv_den.ptr, inv.data(), sizeof(float) * inv.size(), hipMemcpyHostToDevice), "upload") &&
+         hip_ok(hipMemcpy(bc->lag.ptr, lag.data(), sizeof(float) * lag.size(), hipMemcpyHostToDevice), "upload") &&
          hip_ok(hipMemcpy(bc->bininfo.ptr, binfo.data(), sizeof(float4) * nb, hipMemcpyHostToDevice), "upload");
 }
 

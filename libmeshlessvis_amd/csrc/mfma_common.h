@@ -60,6 +60,38 @@ __device__ __forceinline__ void unpack2(uint32_t u, float* a, float* b) {
   }
 }
 
+// Packet load through a global-address-space pointer (a global load, which the
+// wait counters track apart from LDS traffic, instead of a flat load).
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ Packet load_packet(const Packet* p) {
+  const u32x4_t v = *(const __attribute__((address_space(1))) u32x4_t*)p;
+  Packet q;
+  q.pu = v[0];
+  q.pv = v[1];
+  // Elements are copied to scalars before the bit cast: clang's
+  // __builtin_bit_cast of a vector-element lvalue reads element 0.
+  const uint32_t wb = v[2], hb = v[3];
+  q.w = __builtin_bit_cast(float, wb);
+  q.h = __builtin_bit_cast(float, hb);
+  return q;
+}
+__device__ __forceinline__ float4 load_float4(const float4* p) {
+  const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t*)p;
+  return make_float4(v[0], v[1], v[2], v[3]);
+}
+
+// Half an LSB of the 23 phase bits phase_rev keeps; added once to a phase so
+// that phase_rev rounds instead of truncating.
+constexpr uint32_t kPhaseRound = 1u << 8;
+
+// Fixed-point phase P (revolutions * 2^32) -> 1 + P / 2^32 rounded down to 23
+// bits, a float in [1, 2): its fraction is the phase, and v_sin/v_cos take
+// revolutions, so sin_rev/cos_rev of it are the sine/cosine of the phase.  One
+// v_alignbit_b32: the exponent bits of 1.0 (0x7f << 23) above P's top 23 bits.
+__device__ __forceinline__ float phase_rev(uint32_t P) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7fu, P, 9));
+}
+
 template <int OP>
 __device__ __forceinline__ f32x16_t mfma(u32x4_t a, u32x4_t b, f32x16_t c) {
   if (OP == OP_F16)
@@ -80,20 +112,23 @@ __device__ __forceinline__ u32x4_t swap_halves(u32x4_t v) {  // (r, i) -> (i, r)
 // h-binning of a group (mfma_binned.hip): slot k of the per-view packet array
 // holds particle slot_map[k] (-1: zero padding); every bin starts on a
 // 16-particle step; step_bin gives each step's bin.
+// float4 words per slot holding the T Lagrange weights of a particle.
+constexpr int lag_words(int T) { return (T + 3) / 4; }
+
 struct BinCache {
   int n = 0, nb = 0, T = 0;
   int n_slots = 0;
   DevBuf slot_map;  // int[n_slots]
   DevBuf step_bin;  // uint16_t[n_slots / 16]
   DevBuf nodes;     // float[nb][kMaxNodes] Chebyshev nodes in h
-  DevBuf inv_den;   // float[nb][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+  DevBuf lag;       // float4[n_slots][lag_words(T)]: Lagrange weights l_t(h_i) / max |w| of the bin, 0 on padding
   DevBuf bininfo;   // float4[nb] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
   std::vector<int> bin_step;  // host: first step of every bin, nb + 1 entries
   void release() {
     slot_map.release();
     step_bin.release();
     nodes.release();
-    inv_den.release();
+    lag.release();
     bininfo.release();
   }
 };

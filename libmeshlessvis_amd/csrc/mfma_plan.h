@@ -9,7 +9,7 @@ struct MfmaJob {
   const float4* bininfo;     // [bin] (h_lo^2, h_hi^2, max |w|, 1 / max |w|); nullptr: bin0
   const uint16_t* step_bin;  // binned kernel: bin of every 16-particle step
   const float* nodes;        // binned kernel: [bin][kMaxNodes] Chebyshev nodes in h
-  const float* inv_den;      // binned kernel: [bin][kMaxNodes] 1 / prod_{s != t} (h_t - h_s)
+  const float4* lag;         // binned kernel: [slot][lag_words(T)] l_t(h) / max |w| of the slot's bin
   float2* partial;           // [chunk][2cx][cy] partial spectra of the job
   float4 bin0;               // the single bin when bininfo is nullptr
   float radius;              // uniform-radius kind: Phi(r * radius)
