@@ -228,10 +228,14 @@ def batched_bench(args) -> None:
         cpu = cpu_baseline(wls[0], threads)
         log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({cpu['sample']})")
 
-    avg_ms = graph_ms / max(1, launches)
-    achieved = FLOPS_PER_TERM * terms / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
     for ds in sets:
         cfgs[0].unregister(ds)
+    animation = animation_upload(sets, wls, cfgs)
+    log(f"animation with uploads ({animation['timesteps']} timesteps, {animation['upload_bytes'] / 1e6:.0f} MB): "
+        f"synchronous registration {animation['sync_s']:.3f} s, asynchronous {animation['async_s']:.3f} s")
+
+    avg_ms = graph_ms / max(1, launches)
+    achieved = FLOPS_PER_TERM * terms / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
     for c in cfgs:
         c.close()
     out = {
@@ -257,8 +261,39 @@ def batched_bench(args) -> None:
                      "frac": achieved / PEAKS["fp16_TFLOPs"] if achieved else None, "traffic": None,
                      "kernel": "mfma_binned_kernel (whole batch graph)", "kernel_ms_per_launch": avg_ms},
         "cpu_baseline": cpu,
+        "animation_upload": animation,
     }
     print(json.dumps(out), flush=True)
+
+
+def animation_upload(sets, wls, cfgs) -> dict:
+    """Timesteps rendered one after another from host data, each registered (H2D) before its renders:
+    synchronous registration vs asynchronous registration of timestep t+1 while t renders.  The
+    32 views of a timestep are one batched call; both loops end with every image on the host."""
+    from libmeshlessvis_amd import host
+    per = len(cfgs) // len(sets)
+
+    def loop(async_upload: bool) -> float:
+        t0 = time.perf_counter()
+        if async_upload:
+            cfgs[0].register_async(sets[0])
+        for t, ds in enumerate(sets):
+            if not async_upload:
+                cfgs[0].register(ds)
+            group = cfgs[t * per:(t + 1) * per]
+            host.render_batched([ds] * per, group)
+            if async_upload and t + 1 < len(sets):
+                cfgs[0].register_async(sets[t + 1])
+            for c in group:
+                c.image()
+            cfgs[0].unregister(ds)
+        return time.perf_counter() - t0
+
+    loop(True)  # warm: bins and graphs of every timestep
+    sync_s, async_s = loop(False), loop(True)
+    upload = sum(wl.n * (16 + (4 if wl.radii is not None else 0)) for wl in wls)
+    return {"timesteps": len(sets), "renders": len(cfgs), "upload_bytes": upload, "sync_s": sync_s,
+            "async_s": async_s}
 
 
 def main() -> None:

@@ -147,6 +147,14 @@ void vis_synchronize(VisConfig* vis_config);
 /* Synchronising D2H copy of the 2*cx*cy complex spectrum (slot 0, layout [y][x0]). */
 void vis_copy_spectrum_to_host(VisConfig* vis_config, float2* h_spectrum);
 
+/* vis_register_meshless_dataset with the host-to-device copies on a library-owned
+   copy stream: returns without waiting for them, and later work on the config's
+   stream waits for them (stream-ordered), so uploading the next timestep overlaps
+   the render of the current one.  The host arrays must stay unchanged until the
+   config's stream has passed this point (vis_synchronize, vis_copy_to_host of a
+   later render, or vis_unregister_meshless_dataset). */
+void vis_register_meshless_dataset_async(VisConfig* vis_config, MeshlessDataset* meshless_dataset);
+
 /* Render many (dataset, config) pairs; same results as calling
    vis_fourier_volume_rendering on each pair.  When every config has its own
    entry and they share a stream, an MFMA form, the Phi mode, cutoff, samples,

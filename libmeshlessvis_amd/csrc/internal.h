@@ -39,6 +39,10 @@ struct GroupInfo {
   float radius = 1.0f;          // the common radius when uniform_radius
   float h_min = 1.0f, h_max = 1.0f;
   float w_absmax = 0.0f;
+  // Unique per registration (set by registry_put): device state derived from a
+  // group (bins, captured graphs) is keyed by it, because a later registration can
+  // reuse the same device addresses.
+  unsigned long long serial = 0;
 };
 void registry_put(const void* key, const GroupInfo& info);
 bool registry_get(const void* key, GroupInfo* info);

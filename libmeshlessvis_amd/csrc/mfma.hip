@@ -43,12 +43,34 @@
 
 namespace mv {
 
+// One stage = 32 particles; a word of LDS holds one component of two consecutive
+// particles, so an LDS row is 16 words (+4 of padding against bank conflicts).
+constexpr int kStage = 32;
+constexpr int kRowPitch = kStage / 2 + 4;
+
+// hi = rnd(v), lo = rnd(v - hi) of two particles' real and imaginary parts, stored as
+// [re_hi, re_lo, im_hi, im_lo] words of row offset o.
+template <int OP, bool SPLIT, int ROWS>
+__device__ __forceinline__ void store_pair(uint32_t (*arr)[ROWS * kRowPitch], int o, const float* re, const float* im) {
+  const uint32_t hr = pack2<OP>(re[0], re[1]), hm = pack2<OP>(im[0], im[1]);
+  arr[0][o] = hr;
+  arr[2][o] = hm;
+  if (SPLIT) {
+    float a0, a1, b0, b1;
+    unpack2<OP>(hr, &a0, &a1);
+    unpack2<OP>(hm, &b0, &b1);
+    arr[1][o] = pack2<OP>(re[0] - a0, re[1] - a1);
+    arr[3][o] = pack2<OP>(im[0] - b0, im[1] - b1);
+  }
+}
+
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
 __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
                                                                  const MfmaItem* __restrict__ items, MfmaShared sh,
                                                                  int* __restrict__ counter) {
-  // [buffer][A_hi, A_lo, B_hi, B_lo][row * kPitch + particle], one packed (re, im) pair per word
-  __shared__ uint32_t lds[2][4][kTileX * kPitch];
+  // [buffer][re_hi, re_lo, im_hi, im_lo][row * kRowPitch + particle pair]; A rows are |x|, B rows are y.
+  __shared__ uint32_t ldsA[2][4][kTileX * kRowPitch];
+  __shared__ uint32_t ldsB[2][4][kTileY * kRowPitch];
   __shared__ int next_item;
 
   const int tid = threadIdx.x;
@@ -67,165 +89,210 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
     if (it >= sh.n_items) break;
     const MfmaItem item = items[it];
     const MfmaJob job = jobs[item.job];
-    const int j0 = (item.tile % job.tiles_x) * kTileX;
+    const int tile_x = item.tile % job.tiles_x;
+    const int xa0 = 1 + tile_x * kTileX;  // first |x| of the tile
     const int y0 = (item.tile / job.tiles_x) * kTileY;
     const float4 bi = job.bininfo ? job.bininfo[item.bin] : job.bin0;
     const float hh_ref = sh.ref_hi ? bi.y : bi.x;
+    // Particles [p0, p1) of the item, in stages of 32 (the last one may be partly empty:
+    // its missing particles are zero packets, whose B operands are 0).
+    const int p0 = item.s0 * kStep, p1 = item.s1 * kStep;
+    const int n_stages = (p1 - p0 + kStage - 1) / kStage;
 
     if (wave < kProducerWaves) {
-      // ---- producer: operands of step s+1 while the consumers multiply step s ----
+      // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
       const int ptid = role_idx * 64 + lane;
-      const int gp = ptid & (kStep - 1);  // particle of the step
-      const int gg = ptid >> 4;           // 0..15: rows / columns gg + 16m
-      // Row gg + 16m of the tile: phase = P0 + m * 16 * (phase step), exact in
-      // 32-bit modular arithmetic.
-      const uint32_t x_first = static_cast<uint32_t>(j0 + gg - sh.cx + 1);
+      const int pp = ptid & 15;  // particles 2pp, 2pp+1 of the stage
+      const int gg = ptid >> 4;  // 0..15: A rows gg + 16m (m < 4), B rows gg + 16m (m < 8)
+      // Row gg + 16m: phase = P0 + m * 16 * (phase step), exact in 32-bit modular arithmetic.
+      const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
       const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
-      float ca[8], cb[8];
+      float ca[4], cb[8];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const float fu = sh.sx * static_cast<float>(static_cast<int>(xa_first) + 16 * m);
+        ca[m] = sh.sigma_pi_log2e * fu * fu;
+      }
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
-        const float fu = sh.sx * static_cast<float>(static_cast<int>(x_first) + 16 * m);
         const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
-        ca[m] = sh.sigma_pi_log2e * fu * fu;
         cb[m] = sh.sigma_pi_log2e * fv * fv;
       }
-      const Packet* pk = job.packets + gp;
-      auto generate = [&](const Packet& q, int buf) {
-        // h_ref is the bin edge that keeps the exponents <= 0 (|alpha|, |beta| <= 1).
-        float dh = q.h * q.h - hh_ref;
-        dh = sh.ref_hi ? fminf(dh, 0.0f) : fmaxf(dh, 0.0f);
-        const float wq = q.w * bi.w;
-        uint32_t PA = x_first * q.pu + kPhaseRound, PB = y_first * q.pv + kPhaseRound;
-        const uint32_t dA = q.pu << 4, dB = q.pv << 4;
+      // x = 0: A(0, i) = 1, so S(0, y) = E(0, y) * sum_i B(i, y), summed here in FP32 by the
+      // tiles of the first |x| range.
+      const bool zero_col = tile_x == 0;
+      float zr[8], zi[8];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
-          const int o = (gg + 16 * m) * kPitch + gp;
-          {  // A(x, i)
-            const float ph = phase_rev(PA);
-            PA += dA;
-            float re = cos_rev(ph), im = sin_rev(ph);
+      for (int m = 0; m < 8; ++m) zr[m] = zi[m] = 0.0f;
+      auto fetch = [&](int k, Packet* q) {
+        const int p = p0 + k * kStage + 2 * pp;
+        q[0] = p < p1 ? load_packet(job.packets + p) : Packet{0u, 0u, 0.0f, 0.0f};
+        q[1] = p + 1 < p1 ? load_packet(job.packets + p + 1) : Packet{0u, 0u, 0.0f, 0.0f};
+      };
+      auto generate = [&](const Packet* q, int buf) {
+        float dh[2], wq[2];
+        uint32_t PA[2], PB[2], dA[2], dB[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          // h_ref is the bin edge that keeps the exponents <= 0 (|alpha|, |beta| <= 1).
+          const float d = q[u].h * q[u].h - hh_ref;
+          dh[u] = sh.ref_hi ? fminf(d, 0.0f) : fmaxf(d, 0.0f);
+          wq[u] = q[u].w * bi.w;
+          PA[u] = xa_first * q[u].pu + kPhaseRound;
+          PB[u] = y_first * q[u].pv + kPhaseRound;
+          dA[u] = q[u].pu << 4;
+          dB[u] = q[u].pv << 4;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {  // A(|x|, i)
+          float re[2], im[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const float ph = phase_rev(PA[u]);
+            PA[u] += dA[u];
+            re[u] = cos_rev(ph);
+            im[u] = sin_rev(ph);
             if (GAUSS) {
-              const float amp = __builtin_amdgcn_exp2f(ca[m] * dh);
-              re *= amp;
-              im *= amp;
-            }
-            const uint32_t hi = pack2<OP>(re, im);
-            lds[buf][0][o] = hi;
-            if (SPLIT) {
-              float hr, hm;
-              unpack2<OP>(hi, &hr, &hm);
-              lds[buf][1][o] = pack2<OP>(re - hr, im - hm);
+              const float amp = __builtin_amdgcn_exp2f(ca[m] * dh[u]);
+              re[u] *= amp;
+              im[u] *= amp;
             }
           }
-          {  // B(i, y)
-            const float ph = phase_rev(PB);
-            PB += dB;
-            float amp = wq;
-            if (GAUSS) amp *= __builtin_amdgcn_exp2f(cb[m] * dh);
-            const float re = amp * cos_rev(ph), im = amp * sin_rev(ph);
-            const uint32_t hi = pack2<OP>(re, im);
-            lds[buf][2][o] = hi;
-            if (SPLIT) {
-              float hr, hm;
-              unpack2<OP>(hi, &hr, &hm);
-              lds[buf][3][o] = pack2<OP>(re - hr, im - hm);
-            }
+          store_pair<OP, SPLIT, kTileX>(ldsA[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
+        }
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {  // B(i, y)
+          float re[2], im[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const float ph = phase_rev(PB[u]);
+            PB[u] += dB[u];
+            float amp = wq[u];
+            if (GAUSS) amp *= __builtin_amdgcn_exp2f(cb[m] * dh[u]);
+            re[u] = amp * cos_rev(ph);
+            im[u] = amp * sin_rev(ph);
           }
+          if (zero_col) {
+            zr[m] += re[0] + re[1];
+            zi[m] += im[0] + im[1];
+          }
+          store_pair<OP, SPLIT, kTileY>(ldsB[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
         }
       };
-      // Packets are loaded one step ahead of their use, so the load's latency
-      // hides behind a whole step of operand generation.
-      Packet qn{};
-      if (item.s0 < item.s1) {
-        const Packet q0 = load_packet(pk + static_cast<size_t>(item.s0) * kStep);
-        if (item.s0 + 1 < item.s1) qn = load_packet(pk + static_cast<size_t>(item.s0 + 1) * kStep);
+      // Packets are loaded one stage ahead of their use, so the load's latency
+      // hides behind a whole stage of operand generation.
+      Packet qn[2] = {};
+      if (n_stages > 0) {
+        Packet q0[2];
+        fetch(0, q0);
+        if (n_stages > 1) fetch(1, qn);
         generate(q0, 0);
       }
       __syncthreads();
-      for (int s = item.s0; s < item.s1; ++s) {
-        if (s + 1 < item.s1) {
-          const Packet q = qn;
-          if (s + 2 < item.s1) qn = load_packet(pk + static_cast<size_t>(s + 2) * kStep);
-          generate(q, (s - item.s0 + 1) & 1);
+      for (int k = 0; k < n_stages; ++k) {
+        if (k + 1 < n_stages) {
+          const Packet q[2] = {qn[0], qn[1]};
+          if (k + 2 < n_stages) fetch(k + 2, qn);
+          generate(q, (k + 1) & 1);
         }
         __syncthreads();
+      }
+      if (zero_col) {
+        // Sum over the 16 lanes of a row group (pp = 0..15), then one lane per row writes.
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+#pragma unroll
+          for (int d = 1; d < 16; d <<= 1) {
+            zr[m] += __shfl_xor(zr[m], d);
+            zi[m] += __shfl_xor(zi[m], d);
+          }
+          const int y = static_cast<int>(y_first) + 16 * m;
+          if (pp == 0 && y < sh.cy) {
+            const float fv = sh.sy * static_cast<float>(y);
+            float e;
+            if (GAUSS)
+              e = __builtin_amdgcn_exp2f(sh.sigma_pi_log2e * hh_ref * fv * fv) * bi.z;
+            else
+              e = phi<BASIS, MODE>(fabsf(fv) * job.radius) * bi.z;
+            job.partial[(static_cast<size_t>(item.chunk) * two_cx + (sh.cx - 1)) * sh.cy + y] =
+                make_float2(e * zr[m], e * zi[m]);
+          }
+        }
       }
       continue;
     }
 
-    // ---- consumer: a 64 (x) x 64 (y) quarter of the tile, four 32x32 blocks ----
+    // ---- consumer: 32 |x| rows x 64 y columns; real products rr, ii, ri, ir ----
     const int wx = role_idx & 1;
     const int wy = role_idx >> 1;
-    f32x16_t acc[2][2][2];  // [row block][column block][re, im]
+    f32x16_t acc[4][2];  // [Ar.Br, Ai.Bi, Ar.Bi, Ai.Br][column block]
 #pragma unroll
-    for (int rb = 0; rb < 2; ++rb)
-#pragma unroll
-      for (int cb = 0; cb < 2; ++cb) acc[rb][cb][0] = acc[rb][cb][1] = (f32x16_t){};
+    for (int q = 0; q < 4; ++q) acc[q][0] = acc[q][1] = (f32x16_t){};
 
     __syncthreads();
-    for (int s = item.s0; s < item.s1; ++s) {
-      const int buf = (s - item.s0) & 1;
+    for (int k = 0; k < n_stages; ++k) {
+      const int buf = k & 1;
 #pragma unroll
-      for (int kk = 0; kk < kStep / 8; ++kk) {
+      for (int kk = 0; kk < kStage / 16; ++kk) {
         const int pofs = kk * 8 + (lane >> 5) * 4;
-        u32x4_t bre_hi[2], bim_hi[2], bre_lo[2], bim_lo[2];
+        const int row = wx * 32 + (lane & 31);
+        const u32x4_t ar = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kRowPitch + pofs]);
+        const u32x4_t ai = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][2][row * kRowPitch + pofs]);
+        u32x4_t arl, ail;
+        if (SPLIT) {
+          arl = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kRowPitch + pofs]);
+          ail = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][3][row * kRowPitch + pofs]);
+        }
 #pragma unroll
         for (int cb = 0; cb < 2; ++cb) {
           const int col = wy * 64 + cb * 32 + (lane & 31);
-          const u32x4_t bhi = *reinterpret_cast<const u32x4_t*>(&lds[buf][2][col * kPitch + pofs]);
-          bre_hi[cb] = negate_imag(bhi);
-          bim_hi[cb] = swap_halves(bhi);
+          const u32x4_t br = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][0][col * kRowPitch + pofs]);
+          const u32x4_t bim = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][2][col * kRowPitch + pofs]);
+          acc[0][cb] = mfma<OP>(ar, br, acc[0][cb]);
+          acc[1][cb] = mfma<OP>(ai, bim, acc[1][cb]);
+          acc[2][cb] = mfma<OP>(ar, bim, acc[2][cb]);
+          acc[3][cb] = mfma<OP>(ai, br, acc[3][cb]);
           if (SPLIT) {
-            const u32x4_t blo = *reinterpret_cast<const u32x4_t*>(&lds[buf][3][col * kPitch + pofs]);
-            bre_lo[cb] = negate_imag(blo);
-            bim_lo[cb] = swap_halves(blo);
-          }
-        }
-#pragma unroll
-        for (int rb = 0; rb < 2; ++rb) {
-          const int row = wx * 64 + rb * 32 + (lane & 31);
-          const u32x4_t ahi = *reinterpret_cast<const u32x4_t*>(&lds[buf][0][row * kPitch + pofs]);
-          u32x4_t alo;
-          if (SPLIT) alo = *reinterpret_cast<const u32x4_t*>(&lds[buf][1][row * kPitch + pofs]);
-#pragma unroll
-          for (int cb = 0; cb < 2; ++cb) {
-            acc[rb][cb][0] = mfma<OP>(ahi, bre_hi[cb], acc[rb][cb][0]);
-            acc[rb][cb][1] = mfma<OP>(ahi, bim_hi[cb], acc[rb][cb][1]);
-            if (SPLIT) {
-              acc[rb][cb][0] = mfma<OP>(ahi, bre_lo[cb], acc[rb][cb][0]);
-              acc[rb][cb][1] = mfma<OP>(ahi, bim_lo[cb], acc[rb][cb][1]);
-              acc[rb][cb][0] = mfma<OP>(alo, bre_hi[cb], acc[rb][cb][0]);
-              acc[rb][cb][1] = mfma<OP>(alo, bim_hi[cb], acc[rb][cb][1]);
-            }
+            const u32x4_t brl = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][1][col * kRowPitch + pofs]);
+            const u32x4_t bil = *reinterpret_cast<const u32x4_t*>(&ldsB[buf][3][col * kRowPitch + pofs]);
+            acc[0][cb] = mfma<OP>(ar, brl, acc[0][cb]);
+            acc[1][cb] = mfma<OP>(ai, bil, acc[1][cb]);
+            acc[2][cb] = mfma<OP>(ar, bil, acc[2][cb]);
+            acc[3][cb] = mfma<OP>(ai, brl, acc[3][cb]);
+            acc[0][cb] = mfma<OP>(arl, br, acc[0][cb]);
+            acc[1][cb] = mfma<OP>(ail, bim, acc[1][cb]);
+            acc[2][cb] = mfma<OP>(arl, bim, acc[2][cb]);
+            acc[3][cb] = mfma<OP>(ail, br, acc[3][cb]);
           }
         }
       }
       __syncthreads();
     }
 
-    // Epilogue: partial[chunk][j][y] = E_b(x, y) * acc,
-    // E_b = exp(s pi h_b^2 r^2) W_b (Gaussian) or Phi(r h) W (uniform radius).
+    // Epilogue: A(-x, i) = conj(A(x, i)), so with P = sum A B split into its real products
+    //   S(+x, y) = E (Prr - Pii, Pri + Pir),  S(-x, y) = E (Prr + Pii, Pri - Pir),
+    // E = exp(s pi h_b^2 r^2) W_b (Gaussian) or Phi(r h) W (uniform radius); partial[chunk][j][y]
+    // with j = x + cx - 1.  x = -cx is outside the half plane.
 #pragma unroll
     for (int cb = 0; cb < 2; ++cb) {
       const int y = y0 + wy * 64 + cb * 32 + (lane & 31);
       if (y >= sh.cy) continue;
       const float fv = sh.sy * static_cast<float>(y);
 #pragma unroll
-      for (int rb = 0; rb < 2; ++rb) {
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const int j = j0 + wx * 64 + rb * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-          if (j >= two_cx) continue;
-          const float fu = sh.sx * static_cast<float>(j - sh.cx + 1);
-          const float r2 = fu * fu + fv * fv;
-          float e;
-          if (GAUSS)
-            e = __builtin_amdgcn_exp2f(sh.sigma_pi_log2e * hh_ref * r2) * bi.z;
-          else
-            e = phi<BASIS, MODE>(sqrtf(r2) * job.radius) * bi.z;
-          job.partial[(static_cast<size_t>(item.chunk) * two_cx + j) * sh.cy + y] =
-              make_float2(e * acc[rb][cb][0][i], e * acc[rb][cb][1][i]);
-        }
+      for (int i = 0; i < 16; ++i) {
+        const int xa = xa0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        if (xa > sh.cx) continue;
+        const float fu = sh.sx * static_cast<float>(xa);
+        const float r2 = fu * fu + fv * fv;
+        float e;
+        if (GAUSS)
+          e = __builtin_amdgcn_exp2f(sh.sigma_pi_log2e * hh_ref * r2) * bi.z;
+        else
+          e = phi<BASIS, MODE>(sqrtf(r2) * job.radius) * bi.z;
+        const float prr = acc[0][cb][i], pii = acc[1][cb][i], pri = acc[2][cb][i], pir = acc[3][cb][i];
+        float2* col = job.partial + static_cast<size_t>(item.chunk) * two_cx * sh.cy + y;
+        col[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(e * (prr - pii), e * (pri + pir));
+        if (xa < sh.cx) col[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(e * (prr + pii), e * (pri - pir));
       }
     }
   }
@@ -360,8 +427,11 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   // kernel: ranges may cross bins (it folds at bin changes).  Sized so that a single
   // render still fills the chip several times over.
   const bool binned = out->kind == KIND_BINNED;
-  const int tx = binned ? kBinnedTileX : kTileX, ty = binned ? kBinnedTileY : kTileY;
-  job.tiles_x = (2 * spec.vp.cx + tx - 1) / tx;
+  // Sum kernel tiles cover |x| in [1, cx] (each row gives x and -x; x = 0 comes with the
+  // first range); binned kernel tiles cover j = x + cx - 1 in [0, 2cx).
+  job.tiles_x = binned ? (2 * spec.vp.cx + kBinnedTileX - 1) / kBinnedTileX
+                       : std::max(1, (spec.vp.cx + kTileX - 1) / kTileX);
+  const int ty = binned ? kBinnedTileY : kTileY;
   const int tiles = job.tiles_x * ((spec.vp.cy + ty - 1) / ty);
   const long long total_steps = bin_step.back();
   const int cap = binned ? 4096 : kMaxChunkSteps;

@@ -44,8 +44,11 @@ static std::unordered_map<const void*, GroupInfo>& reg() {
   return m;
 }
 void registry_put(const void* key, const GroupInfo& info) {
+  static unsigned long long next_serial = 0;
   std::lock_guard<std::mutex> lk(g_reg_mu);
-  reg()[key] = info;
+  GroupInfo& e = reg()[key];
+  e = info;
+  e.serial = ++next_serial;
 }
 bool registry_get(const void* key, GroupInfo* info) {
   std::lock_guard<std::mutex> lk(g_reg_mu);
@@ -111,6 +114,7 @@ struct VisPlan {
   mv::DevBuf acc, partial, packets;
   mv::MfmaWorkspace mfma;
   BatchGraph batch;
+  hipStream_t copy_stream = nullptr;  // asynchronous registration (created on first use)
   bool timing = false;
   std::vector<mv::EventPair> events;
   size_t events_used = 0;
@@ -146,6 +150,10 @@ static void plan_destroy(VisPlan* p) {
   p->packets.release();
   p->mfma.release();
   p->batch.release();
+  if (p->copy_stream) {
+    (void)hipStreamSynchronize(p->copy_stream);
+    (void)hipStreamDestroy(p->copy_stream);
+  }
   for (auto& e : p->events) {
     (void)hipEventDestroy(e.first);
     (void)hipEventDestroy(e.second);
@@ -398,6 +406,11 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
     key_put(&k, ds[i]->number_of_groups);
     for (int g = 0; g < ds[i]->number_of_groups; ++g) {
       const Group& gr = ds[i]->groups[g];
+      // The registration serial: a group unregistered and registered again (possibly at
+      // the same device addresses) has new bins, so its old graph must not be replayed.
+      GroupInfo info;
+      const unsigned long long serial = registry_get(gr.d_constraints, &info) ? info.serial : 0ull;
+      k.push_back(serial);
       key_put(&k, gr.d_constraints);
       key_put(&k, gr.d_radii);
       key_put(&k, gr.number_of_terms);
@@ -587,18 +600,26 @@ void vis_config_compute_scale(VisConfig* c) {
 }
 
 // Registration: zero-padded AoS copies, as load_into_device (meshless_vis.cu:264-290).
-void vis_register_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+// Synchronous: copies on the config's stream, then waits for it.  Asynchronous:
+// copies on the plan's copy stream; the config's stream waits for them through an
+// event, the host does not.
+static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const char* what) {
   clear_error();
   if (!c || !ds) {
-    set_error(VIS_ERROR_INVALID_VALUE, "vis_register_meshless_dataset: null argument");
+    set_error(VIS_ERROR_INVALID_VALUE, std::string(what) + ": null argument");
     return;
   }
   const long long multiple = static_cast<long long>(c->_number_of_partial_sums) * c->block_length;
   if (multiple <= 0) {
-    set_error(VIS_ERROR_INVALID_CONFIG, "vis_register_meshless_dataset: number_of_partial_sums*block_length <= 0");
+    set_error(VIS_ERROR_INVALID_CONFIG, std::string(what) + ": number_of_partial_sums*block_length <= 0");
     return;
   }
-  hipStream_t s = c->_plan->stream;
+  VisPlan* p = c->_plan;
+  if (async && !p->copy_stream &&
+      !hip_ok(hipStreamCreateWithFlags(&p->copy_stream, hipStreamNonBlocking), "hipStreamCreate(copy)"))
+    return;
+  hipStream_t s = async ? p->copy_stream : p->stream;
+  std::vector<std::pair<const void*, GroupInfo>> infos;
   for (int j = 0; j < ds->number_of_groups; ++j) {
     Group& g = ds->groups[j];
     g.d_constraints = nullptr;
@@ -644,9 +665,27 @@ void vis_register_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
       info.uniform_radius = true;
       info.radius = 1.0f;
     }
-    if (!hip_ok(hipStreamSynchronize(s), "register synchronize")) return;
-    if (g.d_constraints) registry_put(g.d_constraints, info);
+    if (g.d_constraints) infos.emplace_back(g.d_constraints, info);
   }
+  if (async) {
+    hipEvent_t ev = nullptr;
+    if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate")) return;
+    const bool ok = hip_ok(hipEventRecord(ev, s), "hipEventRecord") &&
+                    hip_ok(hipStreamWaitEvent(p->stream, ev, 0), "hipStreamWaitEvent");
+    (void)hipEventDestroy(ev);
+    if (!ok) return;
+  } else if (!hip_ok(hipStreamSynchronize(s), "register synchronize")) {
+    return;
+  }
+  for (const auto& e : infos) registry_put(e.first, e.second);
+}
+
+void vis_register_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+  register_groups(c, ds, false, "vis_register_meshless_dataset");
+}
+
+void vis_register_meshless_dataset_async(VisConfig* c, MeshlessDataset* ds) {
+  register_groups(c, ds, true, "vis_register_meshless_dataset_async");
 }
 
 void vis_unregister_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {

@@ -154,6 +154,12 @@ class Config:
         lib().vis_register_meshless_dataset(self.p, ds.ptr)
         check("vis_register_meshless_dataset")
 
+    def register_async(self, ds: Dataset) -> None:
+        """vis_register_meshless_dataset_async: ``ds``'s host arrays must stay alive and
+        unchanged until this config's stream has passed the copies."""
+        lib().vis_register_meshless_dataset_async(self.p, ds.ptr)
+        check("vis_register_meshless_dataset_async")
+
     def unregister(self, ds: Dataset) -> None:
         lib().vis_unregister_meshless_dataset(self.p, ds.ptr)
         check("vis_unregister_meshless_dataset")

@@ -109,3 +109,27 @@ def test_mixed_forms_fall_back_to_sequential():
         a.unregister(ds)
     a.close()
     b.close()
+
+
+def test_batched_graph_after_reregistration():
+    """A dataset unregistered and registered again gets new bins (its device arrays may come back
+    at the same addresses): the batch graph captured for the old registration must not be replayed."""
+    rng = np.random.default_rng(31)
+    ds = make_sets(rng)[1]  # Wendland with radii: binned kernel, per-registration bins
+    vws = views(rng, 2)
+    cfgs = [host.Config(cutoff=(64, 32), samples=(128, 96), step=(0.5, 0.5), u=vw.u, v=vw.v,
+                        form="mfma_split_f16", phi_mode="analytic") for vw in vws]
+    eager = []
+    for rep in range(3):
+        cfgs[0].register(ds)
+        host.render_batched([ds, ds], cfgs)
+        imgs = [c.image() for c in cfgs]
+        if not eager:
+            for c in cfgs:
+                c.render(ds)
+                eager.append(c.image())
+        for a, b in zip(imgs, eager):
+            assert oracle.rel_l2(a, b) <= 1e-6, rep
+        cfgs[0].unregister(ds)
+    for c in cfgs:
+        c.close()
