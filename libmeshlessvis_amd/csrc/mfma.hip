@@ -48,22 +48,6 @@ namespace mv {
 constexpr int kStage = 32;
 constexpr int kRowPitch = kStage / 2 + 4;
 
-// hi = rnd(v), lo = rnd(v - hi) of two particles' real and imaginary parts, stored as
-// [re_hi, re_lo, im_hi, im_lo] words of row offset o.
-template <int OP, bool SPLIT, int ROWS>
-__device__ __forceinline__ void store_pair(uint32_t (*arr)[ROWS * kRowPitch], int o, const float* re, const float* im) {
-  const uint32_t hr = pack2<OP>(re[0], re[1]), hm = pack2<OP>(im[0], im[1]);
-  arr[0][o] = hr;
-  arr[2][o] = hm;
-  if (SPLIT) {
-    float a0, a1, b0, b1;
-    unpack2<OP>(hr, &a0, &a1);
-    unpack2<OP>(hm, &b0, &b1);
-    arr[1][o] = pack2<OP>(re[0] - a0, re[1] - a1);
-    arr[3][o] = pack2<OP>(im[0] - b0, im[1] - b1);
-  }
-}
-
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
 __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
                                                                  const MfmaItem* __restrict__ items, MfmaShared sh,
@@ -120,7 +104,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
       }
       // x = 0: A(0, i) = 1, so S(0, y) = E(0, y) * sum_i B(i, y), summed here in FP32 by the
       // tiles of the first |x| range.
-      const bool zero_col = tile_x == 0;
+      const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
       float zr[8], zi[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) zr[m] = zi[m] = 0.0f;
@@ -158,7 +142,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
               im[u] *= amp;
             }
           }
-          store_pair<OP, SPLIT, kTileX>(ldsA[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
+          store_pair<OP, SPLIT, kTileX * kRowPitch>(ldsA[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
         }
 #pragma unroll
         for (int m = 0; m < 8; ++m) {  // B(i, y)
@@ -176,7 +160,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
             zr[m] += re[0] + re[1];
             zi[m] += im[0] + im[1];
           }
-          store_pair<OP, SPLIT, kTileY>(ldsB[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
+          store_pair<OP, SPLIT, kTileY * kRowPitch>(ldsB[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
         }
       };
       // Packets are loaded one stage ahead of their use, so the load's latency
@@ -399,13 +383,14 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
       const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 1.25;
       nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
     } else {
-      T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 3;
+      T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 2;
       if (T < 2 || T > kMaxNodes) {
         set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
         return false;
       }
-      // Defaults: 3 Chebyshev nodes per bin of ratio 1.04 (interpolation error < 3e-6 of Phi(0)).
-      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 3 ? 1.04 : 1.1);
+      // Defaults: 2 Chebyshev nodes per bin of ratio 1.01 (the two-node kernel; interpolation
+      // error < 1.2e-5 of Phi(0) pointwise), 3 nodes: 1.04 (< 3e-6), more: 1.1.
+      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? 1.01 : T == 3 ? 1.04 : 1.1);
       nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
     }
     if (nb > 65535) {
@@ -428,10 +413,13 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   // render still fills the chip several times over.
   const bool binned = out->kind == KIND_BINNED;
   // Sum kernel tiles cover |x| in [1, cx] (each row gives x and -x; x = 0 comes with the
-  // first range); binned kernel tiles cover j = x + cx - 1 in [0, 2cx).
-  job.tiles_x = binned ? (2 * spec.vp.cx + kBinnedTileX - 1) / kBinnedTileX
-                       : std::max(1, (spec.vp.cx + kTileX - 1) / kTileX);
-  const int ty = binned ? kBinnedTileY : kTileY;
+  // first range); binned kernel tiles (T >= 3) cover j = x + cx - 1 in [0, 2cx).
+  // Two-node binned kernel: |x| rows as the sum kernel.
+  const bool paired = binned && out->T == 2;
+  job.tiles_x = !binned ? std::max(1, (spec.vp.cx + kTileX - 1) / kTileX)
+                : paired ? std::max(1, (spec.vp.cx + kPairTileX - 1) / kPairTileX)
+                         : (2 * spec.vp.cx + kBinnedTileX - 1) / kBinnedTileX;
+  const int ty = !binned ? kTileY : paired ? kPairTileY : kBinnedTileY;
   const int tiles = job.tiles_x * ((spec.vp.cy + ty - 1) / ty);
   const long long total_steps = bin_step.back();
   const int cap = binned ? 4096 : kMaxChunkSteps;

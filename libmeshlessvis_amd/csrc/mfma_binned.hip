@@ -239,6 +239,272 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
   }
 }
 
+// ---- Two nodes per bin: x and -x from one |x| row -------------------------
+//
+// A(-x, i) = conj A(x, i), so the four real products of A = (Ar, Ai) and
+// B_t = (Br, Bi) over a bin, Prr = Ar.Br, Pii = Ai.Bi, Pri = Ar.Bi, Pir = Ai.Br,
+// give both S(+x) = (Prr - Pii, Pri + Pir) and S(-x) = (Prr + Pii, Pri - Pir): one
+// |x| row costs the MFMAs of one x row of the kernel above and yields two.  The
+// real and imaginary parts are separate LDS arrays (two particles per word), so
+// the MFMA's K index is the particle.  x = 0 (A = 1) is the column sum of B_t,
+// kept by the lanes that generate B.
+//
+// Tile: 64 |x| rows x 64 columns.  Waves 0-3 multiply node 0, waves 4-7 node 1,
+// each a 32 x 32 block (4 products of accumulators, folded S(+-x) sums); at the end
+// of an item the node-1 waves hand their sums to the node-0 waves through LDS.
+constexpr int kPairPitch = kStep / 2 + 4;  // 8 words of 16 particles + 4 against bank conflicts
+
+template <int OP, bool SPLIT, int BASIS, int MODE>
+__global__ __launch_bounds__(kThreads, 1) void mfma_binned_pair_kernel(const MfmaJob* __restrict__ jobs,
+                                                                         const MfmaItem* __restrict__ items,
+                                                                         MfmaShared sh, int* __restrict__ counter) {
+  constexpr int T = 2;
+  // [buf][re_hi, re_lo, im_hi, im_lo] rows of A (|x|) and, per node, of B (y).
+  __shared__ uint32_t ldsA[2][4][kPairTileX * kPairPitch];
+  __shared__ uint32_t ldsB[2][T][4][kPairTileY * kPairPitch];
+  // End of an item: the node-1 waves' folded sums, [wave & 3][component][i][lane].
+  __shared__ float xchg[4 * 4 * 16 * 64];
+  __shared__ int next_item;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int node = wave >> 2;       // MFMA node of this wave
+  const int wx = wave & 1;          // 32-row |x| block
+  const int wy = (wave >> 1) & 1;   // 32-column block
+  const int pp = tid & 7;           // generation: particles 2pp, 2pp+1 of the step
+  const int gg = tid >> 3;          // generation: A row gg, B row gg (0..63)
+  const int two_cx = 2 * sh.cx;
+
+  for (;;) {
+    if (tid == 0) next_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int it = next_item;
+    __syncthreads();
+    if (it >= sh.n_items) break;
+    const MfmaItem item = items[it];
+    const MfmaJob job = jobs[item.job];
+    const int tile_x = item.tile % job.tiles_x;
+    const int xa0 = 1 + tile_x * kPairTileX;
+    const int y0 = (item.tile / job.tiles_x) * kPairTileY;
+    const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
+    const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
+    const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
+
+    struct Ahead {
+      Packet q[2];
+      float2 l[2];
+    };
+    auto fetch = [&](int step, Ahead* a) {
+      const size_t slot = static_cast<size_t>(step) * kStep + 2 * pp;
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        a->q[u] = load_packet(job.packets + slot + u);
+        const float4 l = load_float4(job.lag + slot + u);
+        a->l[u] = make_float2(l.x, l.y);
+      }
+    };
+    auto step_bin = [&](int step) -> int {
+      return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + step);
+    };
+    float zr[T], zi[T], z0r = 0.0f, z0i = 0.0f;  // x = 0: this lane's column sums of B_t; folded
+#pragma unroll
+    for (int t = 0; t < T; ++t) zr[t] = zi[t] = 0.0f;
+
+    // Also returns this step's x = 0 contributions (its two particles' B_t) in gr, gi.
+    auto generate = [&](const Ahead& a, int buf, float* gr, float* gi) {
+      float re[2], im[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const float ph = phase_rev(xa_first * a.q[u].pu + kPhaseRound);
+        re[u] = cos_rev(ph);
+        im[u] = sin_rev(ph);
+      }
+      store_pair<OP, SPLIT, kPairTileX * kPairPitch>(ldsA[buf], gg * kPairPitch + pp, re, im);
+      float re0[2], im0[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const float ph = phase_rev(y_first * a.q[u].pv + kPhaseRound);
+        re0[u] = a.q[u].w * cos_rev(ph);
+        im0[u] = a.q[u].w * sin_rev(ph);
+      }
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const float l = t == 0 ? a.l[u].x : a.l[u].y;
+          re[u] = l * re0[u];
+          im[u] = l * im0[u];
+        }
+        gr[t] = re[0] + re[1];
+        gi[t] = im[0] + im[1];
+        store_pair<OP, SPLIT, kPairTileY * kPairPitch>(ldsB[buf][t], gg * kPairPitch + pp, re, im);
+      }
+    };
+
+    f32x16_t acc[4], sp[2], sm[2];  // [Prr, Pii, Pri, Pir] of this wave's node; folded S(+x), S(-x)
+    acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
+    sp[0] = sp[1] = sm[0] = sm[1] = (f32x16_t){};
+
+    auto consume = [&](int buf) {
+      const int pofs = (lane >> 5) * 4;
+      const int row = wx * 32 + (lane & 31);
+      const int col = wy * 32 + (lane & 31);
+      const uint32_t* bt = &ldsB[buf][node][0][0];
+      const u32x4_t ar = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kPairPitch + pofs]);
+      const u32x4_t ai = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][2][row * kPairPitch + pofs]);
+      const u32x4_t br = *reinterpret_cast<const u32x4_t*>(bt + 0 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
+      const u32x4_t bi = *reinterpret_cast<const u32x4_t*>(bt + 2 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
+      acc[0] = mfma<OP>(ar, br, acc[0]);
+      acc[1] = mfma<OP>(ai, bi, acc[1]);
+      acc[2] = mfma<OP>(ar, bi, acc[2]);
+      acc[3] = mfma<OP>(ai, br, acc[3]);
+      if (SPLIT) {
+        const u32x4_t arl = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kPairPitch + pofs]);
+        const u32x4_t ail = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][3][row * kPairPitch + pofs]);
+        const u32x4_t brl = *reinterpret_cast<const u32x4_t*>(bt + 1 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
+        const u32x4_t bil = *reinterpret_cast<const u32x4_t*>(bt + 3 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
+        acc[0] = mfma<OP>(ar, brl, acc[0]);
+        acc[1] = mfma<OP>(ai, bil, acc[1]);
+        acc[2] = mfma<OP>(ar, bil, acc[2]);
+        acc[3] = mfma<OP>(ai, brl, acc[3]);
+        acc[0] = mfma<OP>(arl, br, acc[0]);
+        acc[1] = mfma<OP>(ail, bi, acc[1]);
+        acc[2] = mfma<OP>(arl, bi, acc[2]);
+        acc[3] = mfma<OP>(ail, br, acc[3]);
+      }
+    };
+
+    const int y = y0 + wy * 32 + (lane & 31);
+    const float fv = sh.sy * static_cast<float>(y);
+    const float fv0 = sh.sy * static_cast<float>(static_cast<int>(y_first));
+    // S(+-x) += W_b Phi(r h_b,node) S_node(+-x); acc = 0.  Every lane also folds the x = 0
+    // column sums of both nodes (its generation duty).
+    auto fold = [&](int b) {
+      const float* nd = job.nodes + b * kMaxNodes;
+      const float ws = job.bininfo[b].z;
+      const float hn = nd[node];
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int xa = xa0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const float fu = sh.sx * static_cast<float>(xa);
+        const float f = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * hn) * ws;
+        const float prr = acc[0][i], pii = acc[1][i], pri = acc[2][i], pir = acc[3][i];
+        sp[0][i] = fmaf(f, prr - pii, sp[0][i]);
+        sp[1][i] = fmaf(f, pri + pir, sp[1][i]);
+        sm[0][i] = fmaf(f, prr + pii, sm[0][i]);
+        sm[1][i] = fmaf(f, pri - pir, sm[1][i]);
+        __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
+      }
+      acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
+      if (zero_col) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          const float f0 = phi<BASIS, MODE>(fabsf(fv0) * nd[t]) * ws;
+          z0r = fmaf(f0, zr[t], z0r);
+          z0i = fmaf(f0, zi[t], z0i);
+          zr[t] = zi[t] = 0.0f;
+        }
+      }
+    };
+
+    // Packets, Lagrange weights and step bins are loaded one step ahead of their
+    // use, so their latency hides behind a whole step.  The x = 0 contributions of a
+    // step are generated with it and added to the column sums when the step is
+    // multiplied, i.e. after the fold of the previous bin.
+    Ahead an{};
+    float pr[T] = {}, pi[T] = {};  // x = 0 contributions of the step generated last
+    int cur = 0, bn = 0;
+    if (item.s0 < item.s1) {
+      Ahead a0;
+      fetch(item.s0, &a0);
+      cur = step_bin(item.s0);
+      if (item.s0 + 1 < item.s1) {
+        fetch(item.s0 + 1, &an);
+        bn = step_bin(item.s0 + 1);
+      }
+      generate(a0, 0, pr, pi);
+    }
+    __syncthreads();
+    int since = 0;
+    int b = cur;
+    for (int s = item.s0; s < item.s1; ++s) {
+      const int buf = (s - item.s0) & 1;
+      const int b_next = bn;
+      float cr[T], ci[T];  // step s
+#pragma unroll
+      for (int t = 0; t < T; ++t) {
+        cr[t] = pr[t];
+        ci[t] = pi[t];
+      }
+      if (s + 1 < item.s1) {
+        const Ahead a = an;
+        if (s + 2 < item.s1) {
+          fetch(s + 2, &an);
+          bn = step_bin(s + 2);
+        }
+        generate(a, buf ^ 1, pr, pi);
+      }
+      if (b != cur) {
+        fold(cur);
+        cur = b;
+        since = 0;
+      }
+      if (zero_col) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+          zr[t] += cr[t];
+          zi[t] += ci[t];
+        }
+      }
+      consume(buf);
+      if (++since == kBFlushSteps) {
+        fold(cur);
+        since = 0;
+      }
+      b = b_next;
+      __syncthreads();
+    }
+    if (since) fold(cur);
+
+    // Node 1 hands its sums to node 0.
+    if (node == 1) {
+      float* xw = xchg + (wave & 3) * (4 * 16 * 64);
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        xw[(0 * 16 + i) * 64 + lane] = sp[0][i];
+        xw[(1 * 16 + i) * 64 + lane] = sp[1][i];
+        xw[(2 * 16 + i) * 64 + lane] = sm[0][i];
+        xw[(3 * 16 + i) * 64 + lane] = sm[1][i];
+      }
+    }
+    __syncthreads();
+    if (node == 0 && y < sh.cy) {
+      const float* xw = xchg + (wave & 3) * (4 * 16 * 64);
+      float2* out = job.partial + static_cast<size_t>(item.chunk) * two_cx * sh.cy + y;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const int xa = xa0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        if (xa > sh.cx) continue;
+        out[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] =
+            make_float2(sp[0][i] + xw[(0 * 16 + i) * 64 + lane], sp[1][i] + xw[(1 * 16 + i) * 64 + lane]);
+        if (xa < sh.cx)
+          out[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] =
+              make_float2(sm[0][i] + xw[(2 * 16 + i) * 64 + lane], sm[1][i] + xw[(3 * 16 + i) * 64 + lane]);
+      }
+    }
+    if (zero_col) {
+#pragma unroll
+      for (int d = 1; d < 8; d <<= 1) {
+        z0r += __shfl_xor(z0r, d);
+        z0i += __shfl_xor(z0i, d);
+      }
+      if (pp == 0 && static_cast<int>(y_first) < sh.cy)
+        job.partial[(static_cast<size_t>(item.chunk) * two_cx + (sh.cx - 1)) * sh.cy + y_first] = make_float2(z0r, z0i);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ host ---
 
 // Per-group binning, built once per registration and kept until unregister
@@ -394,11 +660,27 @@ static void launch_bt(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaIt
        : launch_b<OP, SPLIT, T, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
 }
 
+template <int OP, bool SPLIT, int BASIS, int MODE>
+static void launch_pair(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
+                        int* counter) {
+  hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE>), dim3(grid), dim3(kThreads), 0, s, jobs, items,
+                     sh, counter);
+}
+
 template <int OP, bool SPLIT>
 static void launch_bf(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
                       int* counter, int T, int basis, int mode) {
+  if (T == 2) {
+    const bool an = mode == VIS_PHI_ANALYTIC;
+    if (basis == SPH)
+      an ? launch_pair<OP, SPLIT, SPH, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+         : launch_pair<OP, SPLIT, SPH, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
+    else
+      an ? launch_pair<OP, SPLIT, WENDLAND_D3_C2, VIS_PHI_ANALYTIC>(grid, s, jobs, items, sh, counter)
+         : launch_pair<OP, SPLIT, WENDLAND_D3_C2, VIS_PHI_REFERENCE>(grid, s, jobs, items, sh, counter);
+    return;
+  }
   switch (T) {
-    case 2: launch_bt<OP, SPLIT, 2>(grid, s, jobs, items, sh, counter, basis, mode); break;
     case 3: launch_bt<OP, SPLIT, 3>(grid, s, jobs, items, sh, counter, basis, mode); break;
     case 4: launch_bt<OP, SPLIT, 4>(grid, s, jobs, items, sh, counter, basis, mode); break;
     default: launch_bt<OP, SPLIT, 5>(grid, s, jobs, items, sh, counter, basis, mode); break;

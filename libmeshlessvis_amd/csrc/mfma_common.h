@@ -92,6 +92,23 @@ __device__ __forceinline__ float phase_rev(uint32_t P) {
   return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7fu, P, 9));
 }
 
+// hi = rnd(v), lo = rnd(v - hi) of two consecutive particles' real and imaginary
+// parts, stored as words [re_hi, re_lo, im_hi, im_lo][o] of an LDS array of N words
+// per part (one 16-bit value of each particle per word).
+template <int OP, bool SPLIT, int N>
+__device__ __forceinline__ void store_pair(uint32_t (*arr)[N], int o, const float* re, const float* im) {
+  const uint32_t hr = pack2<OP>(re[0], re[1]), hm = pack2<OP>(im[0], im[1]);
+  arr[0][o] = hr;
+  arr[2][o] = hm;
+  if (SPLIT) {
+    float a0, a1, b0, b1;
+    unpack2<OP>(hr, &a0, &a1);
+    unpack2<OP>(hm, &b0, &b1);
+    arr[1][o] = pack2<OP>(re[0] - a0, re[1] - a1);
+    arr[3][o] = pack2<OP>(im[0] - b0, im[1] - b1);
+  }
+}
+
 template <int OP>
 __device__ __forceinline__ f32x16_t mfma(u32x4_t a, u32x4_t b, f32x16_t c) {
   if (OP == OP_F16)
@@ -140,6 +157,9 @@ int auto_bins(float h_min, float h_max, double ratio);
 // Tile of the binned kernel (mfma_binned.hip).
 constexpr int kBinnedTileX = 128;
 constexpr int kBinnedTileY = 64;
+// Tile of the two-node binned kernel: |x| rows (x and -x each) x y columns.
+constexpr int kPairTileX = 64;
+constexpr int kPairTileY = 64;
 
 // Persistent binned kernel over a job/item table (mfma_binned.hip); false (error recorded) on failure.
 bool launch_binned(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,

@@ -880,7 +880,8 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
   for (const float2& z : ref) ref_norm += static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
   std::vector<std::pair<int, float>> cand;
   if (binned)
-    cand = {{2, 1.01f}, {2, 1.02f}, {3, 1.03f}, {3, 1.04f}, {3, 1.06f}, {4, 1.1f}, {4, 1.15f}, {5, 1.2f}, {5, 1.3f}};
+    cand = {{2, 1.005f}, {2, 1.01f}, {2, 1.02f}, {3, 1.03f}, {3, 1.04f}, {3, 1.06f}, {4, 1.1f}, {4, 1.15f},
+            {5, 1.2f}, {5, 1.3f}};
   else if (gauss)
     cand = {{nodes0, 1.25f}, {nodes0, 1.5f}, {nodes0, 2.0f}, {nodes0, 4.0f}};
   else

@@ -84,7 +84,7 @@ def test_split_f16_matches_plain_on_clustered_subset():
 
 @pytest.mark.parametrize("form", ["mfma_split_f16", "mfma_split_bf16"])
 @pytest.mark.parametrize("basis", ["WENDLAND_D3_C2", "SPH"])
-@pytest.mark.parametrize("nodes", [3, 4])
+@pytest.mark.parametrize("nodes", [2, 3, 4])
 def test_binned_variable_radii(form, basis, nodes):
     rng = np.random.default_rng(9)
     n = 5000
@@ -93,7 +93,7 @@ def test_binned_variable_radii(form, basis, nodes):
     cutoff, samples, step = (64, 48), (256, 128), (0.5, 0.5)
     ds = host.Dataset(groups)
     with host.Config(cutoff=cutoff, samples=samples, step=step, u=U, v=V, form=form, phi_mode="analytic",
-                     nodes=nodes, h_bins=0 if nodes == 4 else 96) as cfg:
+                     nodes=nodes, h_bins=96 if nodes == 3 else 0) as cfg:
         cfg.register(ds)
         cfg.render(ds)
         F = cfg.spectrum()
