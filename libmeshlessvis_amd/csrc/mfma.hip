@@ -39,6 +39,7 @@
 #include "mfma_common.h"
 
 #include <cstring>
+#include <type_traits>
 #include <tuple>
 
 namespace mv {
@@ -49,26 +50,28 @@ constexpr int kStage = 32;
 constexpr int kRowPitch = kStage / 2 + 4;
 
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
-__global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
-                                                                 const MfmaItem* __restrict__ items, MfmaShared sh,
-                                                                 int* __restrict__ counter) {
+__global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
+                                                                    const MfmaItem* __restrict__ items, MfmaShared sh,
+                                                                    int* __restrict__ counter) {
   // [buffer][re_hi, re_lo, im_hi, im_lo][row * kRowPitch + particle pair]; A rows are |x|, B rows are y.
+  // Together exactly 160 KiB: the work-item index lives in a padding word (words 16..19 of
+  // every row are never read or written by the operand code).
   __shared__ uint32_t ldsA[2][4][kTileX * kRowPitch];
   __shared__ uint32_t ldsB[2][4][kTileY * kRowPitch];
-  __shared__ int next_item;
+  int* next_item = reinterpret_cast<int*>(&ldsA[0][0][kStage / 2]);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  // Waves w and w+4 of a workgroup share a SIMD on gfx950, so every SIMD runs one producer
-  // and one consumer wave and the MFMAs of one overlap the VALU work of the other.
-  const int role_idx = wave & (kProducerWaves - 1);
+  // Waves w, w+4 and w+8 of the workgroup share SIMD w (mod 4): every SIMD runs one
+  // producer and two consumer waves, and the MFMAs of the consumers overlap the VALU
+  // work of the producer.
   const int two_cx = 2 * sh.cx;
 
   for (;;) {
-    if (tid == 0) next_item = atomicAdd(counter, 1);
+    if (tid == 0) *next_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = next_item;
+    const int it = *next_item;
     __syncthreads();
     if (it >= sh.n_items) break;
     const MfmaItem item = items[it];
@@ -85,26 +88,23 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
 
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
-      const int ptid = role_idx * 64 + lane;
+      const int ptid = wave * 64 + lane;
       const int pp = ptid & 15;  // particles 2pp, 2pp+1 of the stage
-      const int gg = ptid >> 4;  // 0..15: A rows gg + 16m (m < 4), B rows gg + 16m (m < 8)
+      const int gg = ptid >> 4;  // 0..15: A rows and B rows gg + 16m (m < 8)
       // Row gg + 16m: phase = P0 + m * 16 * (phase step), exact in 32-bit modular arithmetic.
       const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
       const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
-      float ca[4], cb[8];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const float fu = sh.sx * static_cast<float>(static_cast<int>(xa_first) + 16 * m);
-        ca[m] = sh.sigma_pi_log2e * fu * fu;
-      }
+      float ca[8], cb[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
+        const float fu = sh.sx * static_cast<float>(static_cast<int>(xa_first) + 16 * m);
         const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
+        ca[m] = sh.sigma_pi_log2e * fu * fu;
         cb[m] = sh.sigma_pi_log2e * fv * fv;
       }
       // x = 0: A(0, i) = 1, so S(0, y) = E(0, y) * sum_i B(i, y), summed here in FP32 by the
-      // tiles of the first |x| range.
-      const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
+      // tiles of the first |x| range (cx = 0: an empty half plane).
+      const bool zero_col = tile_x == 0 && sh.cx > 0;
       float zr[8], zi[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) zr[m] = zi[m] = 0.0f;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
         q[0] = p < p1 ? load_packet(job.packets + p) : Packet{0u, 0u, 0.0f, 0.0f};
         q[1] = p + 1 < p1 ? load_packet(job.packets + p + 1) : Packet{0u, 0u, 0.0f, 0.0f};
       };
-      auto generate = [&](const Packet* q, int buf) {
+      auto generate = [&](const Packet* q, int buf, auto zero) {
         float dh[2], wq[2];
         uint32_t PA[2], PB[2], dA[2], dB[2];
 #pragma unroll
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
           dB[u] = q[u].pv << 4;
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {  // A(|x|, i)
+        for (int m = 0; m < 8; ++m) {  // A(|x|, i)
           float re[2], im[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
             re[u] = amp * cos_rev(ph);
             im[u] = amp * sin_rev(ph);
           }
-          if (zero_col) {
+          if (decltype(zero)::value) {
             zr[m] += re[0] + re[1];
             zi[m] += im[0] + im[1];
           }
@@ -165,22 +165,28 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
       };
       // Packets are loaded one stage ahead of their use, so the load's latency
       // hides behind a whole stage of operand generation.
-      Packet qn[2] = {};
-      if (n_stages > 0) {
-        Packet q0[2];
-        fetch(0, q0);
-        if (n_stages > 1) fetch(1, qn);
-        generate(q0, 0);
-      }
-      __syncthreads();
-      for (int k = 0; k < n_stages; ++k) {
-        if (k + 1 < n_stages) {
-          const Packet q[2] = {qn[0], qn[1]};
-          if (k + 2 < n_stages) fetch(k + 2, qn);
-          generate(q, (k + 1) & 1);
+      auto produce = [&](auto zero) {
+        Packet qn[2] = {};
+        if (n_stages > 0) {
+          Packet q0[2];
+          fetch(0, q0);
+          if (n_stages > 1) fetch(1, qn);
+          generate(q0, 0, zero);
         }
         __syncthreads();
-      }
+        for (int k = 0; k < n_stages; ++k) {
+          if (k + 1 < n_stages) {
+            const Packet q[2] = {qn[0], qn[1]};
+            if (k + 2 < n_stages) fetch(k + 2, qn);
+            generate(q, (k + 1) & 1, zero);
+          }
+          __syncthreads();
+        }
+      };
+      if (zero_col)
+        produce(std::true_type{});
+      else
+        produce(std::false_type{});
       if (zero_col) {
         // Sum over the 16 lanes of a row group (pp = 0..15), then one lane per row writes.
 #pragma unroll
@@ -207,8 +213,9 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_sum_kernel(const MfmaJob* __
     }
 
     // ---- consumer: 32 |x| rows x 64 y columns; real products rr, ii, ri, ir ----
-    const int wx = role_idx & 1;
-    const int wy = role_idx >> 1;
+    const int c = wave - kProducerWaves;  // 0..7
+    const int wx = c & 3;
+    const int wy = c >> 2;
     f32x16_t acc[4][2];  // [Ar.Br, Ai.Bi, Ar.Bi, Ai.Br][column block]
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q][0] = acc[q][1] = (f32x16_t){};
@@ -316,7 +323,7 @@ __global__ void zero_ints_kernel(int* p, int n) {
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
 static void launch_sum(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
                        int* counter) {
-  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE>), dim3(grid), dim3(kThreads), 0, s, jobs, items,
+  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE>), dim3(grid), dim3(kSumThreads), 0, s, jobs, items,
                      sh, counter);
 }
 

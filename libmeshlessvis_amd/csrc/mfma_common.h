@@ -15,8 +15,9 @@ typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kStep = 16;          // particles per LDS stage
 constexpr int kPitch = kStep + 4;  // words per LDS row (padding against bank conflicts)
-constexpr int kTileX = 64;         // sum kernel: |x| rows per workgroup tile (x and -x each)
+constexpr int kTileX = 128;        // sum kernel: |x| rows per workgroup tile (x and -x each)
 constexpr int kTileY = 128;        // sum kernel: y columns per workgroup tile
+constexpr int kSumThreads = 768;   // sum kernel: 12 waves, 4 producers and 8 consumers
 constexpr int kThreads = 512;      // 8 waves
 constexpr int kProducerWaves = 4;  // waves 0-3 generate operands, waves 4-7 run the MFMAs
 constexpr int kMaxChunkSteps = 1024;  // at most 16384 particles per FP32 accumulation
