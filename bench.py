@@ -230,9 +230,11 @@ def batched_bench(args) -> None:
 
     for ds in sets:
         cfgs[0].unregister(ds)
-    animation = animation_upload(sets, wls, cfgs)
-    log(f"animation with uploads ({animation['timesteps']} timesteps, {animation['upload_bytes'] / 1e6:.0f} MB): "
-        f"synchronous registration {animation['sync_s']:.3f} s, asynchronous {animation['async_s']:.3f} s")
+    animation = None
+    if not args.no_animation:
+        animation = animation_upload(sets, wls, cfgs)
+        log(f"animation with uploads ({animation['timesteps']} timesteps, {animation['upload_bytes'] / 1e6:.0f} MB): "
+            f"synchronous registration {animation['sync_s']:.3f} s, asynchronous {animation['async_s']:.3f} s")
 
     avg_ms = graph_ms / max(1, launches)
     achieved = FLOPS_PER_TERM * terms / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
@@ -308,6 +310,8 @@ def main() -> None:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-animation", action="store_true",
+                    help="drifting_clusters: skip the per-timestep upload (registration) measurement")
     ap.add_argument("--autotune", type=float, default=0.0, metavar="TOL",
                     help="before timing, let vis_autotune pick nodes/bin ratio within TOL of the plain kernel")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
