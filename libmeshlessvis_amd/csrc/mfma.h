@@ -47,7 +47,11 @@ struct MfmaWorkspace {
   DevBuf counters;  // one work-queue counter per launch
   std::vector<char> tables_host;
   MfmaBatch batch;
+  // What the batch was planned for (groups by registration serial, grid, bin settings): a
+  // render with the same plan reuses the uploaded tables (the plan does not depend on the view).
+  std::vector<unsigned long long> plan_key;
   void release() {
+    plan_key.clear();
     packets.release();
     partial.release();
     tables.release();

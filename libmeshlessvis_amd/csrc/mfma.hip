@@ -433,7 +433,8 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   const int floor_steps = binned ? 128 : 32;
   out->tile_steps = total_steps * tiles;
   if (target_steps <= 0) target_steps = total_steps * tiles / 2048;  // one render: ~2048 items
-  const int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
+  int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
+  chunk_steps += chunk_steps & 1;  // even: items of the two-node kernel are whole 32-particle stages
   int chunk = 0;
   auto add_range = [&](int bin, int s0, int s1) {
     for (int t = 0; t < tiles; ++t) {
@@ -456,12 +457,32 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
         add_range(static_cast<int>(b), s0, std::min(bin_step[b + 1], s0 + chunk_steps));
   }
   if (chunk == 0) add_range(0, 0, 0);
-  out->nchunks = chunk;
+  out->nchunks = paired ? 2 * chunk : chunk;  // the two-node kernel writes one slot per node
   return true;
 }
 
 // Host half of a batch: plans, buffer sizes, work tables uploaded (synchronous).
+static std::vector<unsigned long long> plan_key(const std::vector<MfmaJobSpec>& specs) {
+  std::vector<unsigned long long> k;
+  k.push_back(specs.size());
+  for (const MfmaJobSpec& sp : specs) {
+    uint32_t ratio;
+    std::memcpy(&ratio, &sp.c->h_bin_ratio, sizeof(ratio));
+    for (unsigned long long v :
+         {sp.info.serial, static_cast<unsigned long long>(reinterpret_cast<uintptr_t>(sp.g->d_constraints)),
+          static_cast<unsigned long long>(sp.n), static_cast<unsigned long long>(sp.vp.cx),
+          static_cast<unsigned long long>(sp.vp.cy), static_cast<unsigned long long>(sp.vp.basis),
+          static_cast<unsigned long long>(sp.c->number_of_h_bins),
+          static_cast<unsigned long long>(sp.c->number_of_interpolation_nodes), static_cast<unsigned long long>(ratio)})
+      k.push_back(v);
+  }
+  return k;
+}
+
 bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, MfmaBatch* bt) {
+  std::vector<unsigned long long> key = plan_key(specs);
+  if (bt == &ws->batch && !ws->plan_key.empty() && key == ws->plan_key) return true;  // tables already uploaded
+  ws->plan_key.clear();
   bt->plans.assign(specs.size(), MfmaPlanned{});
   bt->keys.clear();
   bt->job_begin.clear();
@@ -518,17 +539,22 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   bt->job_begin.push_back(static_cast<int>(jobs.size()));
   bt->item_begin.push_back(static_cast<int>(items.size()));
   const size_t jbytes = sizeof(MfmaJob) * jobs.size(), ibytes = sizeof(MfmaItem) * items.size();
+  // The previous upload (and the launches reading the previous tables) must be done before
+  // the host copy and the device tables are reused.
+  if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
   if (!ws->tables.reserve(jbytes + ibytes) || !ws->counters.reserve(sizeof(int) * bt->keys.size())) return false;
   ws->tables_host.resize(jbytes + ibytes);
   std::memcpy(ws->tables_host.data(), jobs.data(), jbytes);
   std::memcpy(ws->tables_host.data() + jbytes, items.data(), ibytes);
-  if (!hip_ok(hipStreamSynchronize(s), "synchronize") ||
-      !hip_ok(hipMemcpy(ws->tables.ptr, ws->tables_host.data(), jbytes + ibytes, hipMemcpyHostToDevice),
+  // Stream-ordered upload: earlier launches on the stream have read the old tables by the time
+  // it runs.  The host copy stays untouched until the next re-plan, which first waits for it.
+  if (!hip_ok(hipMemcpyAsync(ws->tables.ptr, ws->tables_host.data(), jbytes + ibytes, hipMemcpyHostToDevice, s),
               "upload work tables"))
     return false;
   bt->jobs = reinterpret_cast<const MfmaJob*>(ws->tables.ptr);
   bt->items = reinterpret_cast<const MfmaItem*>(static_cast<char*>(ws->tables.ptr) + jbytes);
   bt->counters = ws->counters.as<int>();
+  if (bt == &ws->batch) ws->plan_key = std::move(key);
   return true;
 }
 

@@ -28,6 +28,7 @@
 #include <cmath>
 #include <map>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 namespace mv {
@@ -247,39 +248,48 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 // |x| row costs the MFMAs of one x row of the kernel above and yields two.  The
 // real and imaginary parts are separate LDS arrays (two particles per word), so
 // the MFMA's K index is the particle.  x = 0 (A = 1) is the column sum of B_t,
-// kept by the lanes that generate B.
+// kept by the waves that generate B.
 //
-// Tile: 64 |x| rows x 64 columns.  Waves 0-3 multiply node 0, waves 4-7 node 1,
-// each a 32 x 32 block (4 products of accumulators, folded S(+-x) sums); at the end
-// of an item the node-1 waves hand their sums to the node-0 waves through LDS.
-constexpr int kPairPitch = kStep / 2 + 4;  // 8 words of 16 particles + 4 against bank conflicts
+// Tile: 64 |x| rows x 64 columns, 8 waves.  Waves 0-3 generate the operands of
+// the next 32-particle stage (bins are padded to 32 particles, so a stage lies in
+// one bin); waves 4-7 multiply the current one: consumer c = wave - 4 holds the
+// 32 x 32 block (c % 2, c / 2) for both nodes - 4 products of accumulators per node
+// and the folded S(+-x) sums.  Producer c and consumer c share a SIMD.  The chunk
+// slots of an item are 2 * chunk + t (node t's x = 0 column; the reduction adds
+// the two) - the consumers write both nodes' sum into slot 2 * chunk and zeros'
+// complement is not needed: slot 2 * chunk + 1 holds only its x = 0 column and
+// zeros elsewhere.
+constexpr int kPairStage = 2 * kStep;             // particles per stage
+constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
+constexpr int kPairThreads = 512;
+constexpr int kPairAWords = 2 * 4 * kPairTileX * kPairPitch;      // [buf][re_hi, re_lo, im_hi, im_lo][row]
+constexpr int kPairBWords = 2 * 2 * 4 * kPairTileY * kPairPitch;  // [buf][node][...][row]
 
 template <int OP, bool SPLIT, int BASIS, int MODE>
-__global__ __launch_bounds__(kThreads, 1) void mfma_binned_pair_kernel(const MfmaJob* __restrict__ jobs,
-                                                                         const MfmaItem* __restrict__ items,
-                                                                         MfmaShared sh, int* __restrict__ counter) {
+__global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const MfmaJob* __restrict__ jobs,
+                                                                             const MfmaItem* __restrict__ items,
+                                                                             MfmaShared sh, int* __restrict__ counter) {
   constexpr int T = 2;
-  // [buf][re_hi, re_lo, im_hi, im_lo] rows of A (|x|) and, per node, of B (y).
-  __shared__ uint32_t ldsA[2][4][kPairTileX * kPairPitch];
-  __shared__ uint32_t ldsB[2][T][4][kPairTileY * kPairPitch];
-  // End of an item: the node-1 waves' folded sums, [wave & 3][component][i][lane].
-  __shared__ float xchg[4 * 4 * 16 * 64];
-  __shared__ int next_item;
+  constexpr int kA = kPairTileX * kPairPitch;  // words of one A part
+  constexpr int kB = kPairTileY * kPairPitch;  // words of one B part
+  __shared__ uint32_t lds[kPairAWords + kPairBWords];
+  __shared__ float z0[T][kPairTileY][2];  // x = 0 column of the item per node, folded (LDS atomics)
+  // The work-item index lives in a padding word (words 16..19 of a row are never read or
+  // written by the operand code).
+  int* next_item = reinterpret_cast<int*>(&lds[kPairStage / 2]);
+  auto ldsA = [&](int buf, int part) { return lds + (buf * 4 + part) * kA; };
+  auto ldsB = [&](int buf, int node, int part) { return lds + kPairAWords + ((buf * T + node) * 4 + part) * kB; };
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int node = wave >> 2;       // MFMA node of this wave
-  const int wx = wave & 1;          // 32-row |x| block
-  const int wy = (wave >> 1) & 1;   // 32-column block
-  const int pp = tid & 7;           // generation: particles 2pp, 2pp+1 of the step
-  const int gg = tid >> 3;          // generation: A row gg, B row gg (0..63)
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar register)
   const int two_cx = 2 * sh.cx;
+  for (int i = tid; i < T * kPairTileY * 2; i += kPairThreads) (&z0[0][0][0])[i] = 0.0f;  // ordered by the loop's barrier
 
   for (;;) {
-    if (tid == 0) next_item = atomicAdd(counter, 1);
+    if (tid == 0) *next_item = atomicAdd(counter, 1);
     __syncthreads();
-    const int it = next_item;
+    const int it = *next_item;
     __syncthreads();
     if (it >= sh.n_items) break;
     const MfmaItem item = items[it];
@@ -287,221 +297,267 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_pair_kernel(const Mfm
     const int tile_x = item.tile % job.tiles_x;
     const int xa0 = 1 + tile_x * kPairTileX;
     const int y0 = (item.tile / job.tiles_x) * kPairTileY;
-    const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
-    const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
-    const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
+    // Steps [s0, s1) in stages of two steps; s0 and s1 are even (bins and work items are).
+    const int n_stages = (item.s1 - item.s0) / 2;
+    auto stage_bin = [&](int k) -> int {
+      return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + item.s0 + 2 * k);
+    };
 
-    struct Ahead {
-      Packet q[2];
-      float2 l[2];
-    };
-    auto fetch = [&](int step, Ahead* a) {
-      const size_t slot = static_cast<size_t>(step) * kStep + 2 * pp;
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        a->q[u] = load_packet(job.packets + slot + u);
-        const float4 l = load_float4(job.lag + slot + u);
-        a->l[u] = make_float2(l.x, l.y);
-      }
-    };
-    auto step_bin = [&](int step) -> int {
-      return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + step);
-    };
-    float zr[T], zi[T], z0r = 0.0f, z0i = 0.0f;  // x = 0: this lane's column sums of B_t; folded
-#pragma unroll
-    for (int t = 0; t < T; ++t) zr[t] = zi[t] = 0.0f;
-
-    // Also returns this step's x = 0 contributions (its two particles' B_t) in gr, gi.
-    auto generate = [&](const Ahead& a, int buf, float* gr, float* gi) {
-      float re[2], im[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const float ph = phase_rev(xa_first * a.q[u].pu + kPhaseRound);
-        re[u] = cos_rev(ph);
-        im[u] = sin_rev(ph);
-      }
-      store_pair<OP, SPLIT, kPairTileX * kPairPitch>(ldsA[buf], gg * kPairPitch + pp, re, im);
-      float re0[2], im0[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u) {
-        const float ph = phase_rev(y_first * a.q[u].pv + kPhaseRound);
-        re0[u] = a.q[u].w * cos_rev(ph);
-        im0[u] = a.q[u].w * sin_rev(ph);
-      }
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
+    if (wave < kProducerWaves) {
+      // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
+      const int ptid = wave * 64 + lane;
+      const int pp = ptid & 15;  // particles 2pp, 2pp+1 of the stage
+      const int gg = ptid >> 4;  // 0..15: A rows and B rows gg + 16m (m < 4)
+      const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
+      const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
+      const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
+      struct Ahead {
+        Packet q[2];
+        float2 l[2];
+      };
+      auto fetch = [&](int k, Ahead* a) {
+        const size_t slot = static_cast<size_t>(item.s0) * kStep + k * kPairStage + 2 * pp;
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          const float l = t == 0 ? a.l[u].x : a.l[u].y;
-          re[u] = l * re0[u];
-          im[u] = l * im0[u];
+          a->q[u] = load_packet(job.packets + slot + u);
+          const float4 l = load_float4(job.lag + slot + u);
+          a->l[u] = make_float2(l.x, l.y);
         }
-        gr[t] = re[0] + re[1];
-        gi[t] = im[0] + im[1];
-        store_pair<OP, SPLIT, kPairTileY * kPairPitch>(ldsB[buf][t], gg * kPairPitch + pp, re, im);
+      };
+      // x = 0: this lane's column sums of B_t per column m (current bin); folded into z0.
+      float zr[T][4], zi[T][4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int t = 0; t < T; ++t) zr[t][m] = zi[t][m] = 0.0f;
+      auto zfold = [&](int b) {
+        const float* nd = job.nodes + b * kMaxNodes;
+        const float ws = job.bininfo[b].z;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+            const float f = phi<BASIS, MODE>(fabsf(fv) * nd[t]) * ws;
+            atomicAdd(&z0[t][gg + 16 * m][0], f * zr[t][m]);
+            atomicAdd(&z0[t][gg + 16 * m][1], f * zi[t][m]);
+            zr[t][m] = zi[t][m] = 0.0f;
+          }
+        }
+      };
+      auto generate = [&](const Ahead& a, int buf, auto zero) {
+        uint32_t PA[2], PB[2], dA[2], dB[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          PA[u] = xa_first * a.q[u].pu + kPhaseRound;
+          PB[u] = y_first * a.q[u].pv + kPhaseRound;
+          dA[u] = a.q[u].pu << 4;
+          dB[u] = a.q[u].pv << 4;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {  // A(|x|, i)
+          float re[2], im[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const float ph = phase_rev(PA[u]);
+            PA[u] += dA[u];
+            re[u] = cos_rev(ph);
+            im[u] = sin_rev(ph);
+          }
+          store_pair<OP, SPLIT, kA>(reinterpret_cast<uint32_t(*)[kA]>(ldsA(buf, 0)), (gg + 16 * m) * kPairPitch + pp, re,
+                                    im);
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {  // B_t(i, y)
+          float re0[2], im0[2];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const float ph = phase_rev(PB[u]);
+            PB[u] += dB[u];
+            re0[u] = a.q[u].w * cos_rev(ph);
+            im0[u] = a.q[u].w * sin_rev(ph);
+          }
+#pragma unroll
+          for (int t = 0; t < T; ++t) {
+            float re[2], im[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+              const float l = t == 0 ? a.l[u].x : a.l[u].y;
+              re[u] = l * re0[u];
+              im[u] = l * im0[u];
+            }
+            if (decltype(zero)::value) {
+              zr[t][m] += re[0] + re[1];
+              zi[t][m] += im[0] + im[1];
+            }
+            store_pair<OP, SPLIT, kB>(reinterpret_cast<uint32_t(*)[kB]>(ldsB(buf, t, 0)),
+                                      (gg + 16 * m) * kPairPitch + pp, re, im);
+          }
+        }
+      };
+      // Packets and Lagrange weights are loaded one stage ahead of their use.  The x = 0
+      // sums are folded when the stage about to be generated opens a new bin, and every
+      // kBFlushSteps steps, as the consumers fold theirs.
+      auto produce = [&](auto zero) {
+        Ahead an{};
+        int zb = 0, since = 0;
+        if (n_stages > 0) {
+          Ahead a0;
+          fetch(0, &a0);
+          zb = stage_bin(0);
+          if (n_stages > 1) fetch(1, &an);
+          generate(a0, 0, zero);
+          since = 2;
+        }
+        __syncthreads();
+        for (int k = 0; k < n_stages; ++k) {
+          if (k + 1 < n_stages) {
+            const Ahead a = an;
+            if (k + 2 < n_stages) fetch(k + 2, &an);
+            if (decltype(zero)::value) {
+              const int nb = stage_bin(k + 1);
+              if (nb != zb || since >= kBFlushSteps) {
+                zfold(zb);
+                zb = nb;
+                since = 0;
+              }
+            }
+            generate(a, (k + 1) & 1, zero);
+            since += 2;
+          }
+          __syncthreads();
+        }
+        if (decltype(zero)::value && n_stages > 0) zfold(zb);
+      };
+      if (zero_col)
+        produce(std::true_type{});
+      else
+        produce(std::false_type{});
+      __syncthreads();  // every producer lane's fold is in z0
+      if (zero_col && pp == 0) {
+        // One lane per column writes node t's x = 0 value into chunk slot 2 * chunk + t,
+        // then clears the column for the next item.
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            const int col = gg + 16 * m, y = y0 + col;
+            if (y < sh.cy)
+              job.partial[(static_cast<size_t>(2 * item.chunk + t) * two_cx + (sh.cx - 1)) * sh.cy + y] =
+                  make_float2(z0[t][col][0], z0[t][col][1]);
+            z0[t][col][0] = z0[t][col][1] = 0.0f;
+          }
       }
-    };
+      continue;
+    }
 
-    f32x16_t acc[4], sp[2], sm[2];  // [Prr, Pii, Pri, Pir] of this wave's node; folded S(+x), S(-x)
-    acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
+    // ---- consumer: both nodes of a 32 |x| x 32 y block ----
+    const int c = wave - kProducerWaves;  // 0..3
+    const int wx = c & 1;
+    const int wy = c >> 1;
+    f32x16_t acc[T][4], sp[2], sm[2];  // [node][Prr, Pii, Pri, Pir]; folded S(+x), S(-x)
+#pragma unroll
+    for (int t = 0; t < T; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = (f32x16_t){};
     sp[0] = sp[1] = sm[0] = sm[1] = (f32x16_t){};
-
-    auto consume = [&](int buf) {
-      const int pofs = (lane >> 5) * 4;
-      const int row = wx * 32 + (lane & 31);
-      const int col = wy * 32 + (lane & 31);
-      const uint32_t* bt = &ldsB[buf][node][0][0];
-      const u32x4_t ar = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][0][row * kPairPitch + pofs]);
-      const u32x4_t ai = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][2][row * kPairPitch + pofs]);
-      const u32x4_t br = *reinterpret_cast<const u32x4_t*>(bt + 0 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
-      const u32x4_t bi = *reinterpret_cast<const u32x4_t*>(bt + 2 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
-      acc[0] = mfma<OP>(ar, br, acc[0]);
-      acc[1] = mfma<OP>(ai, bi, acc[1]);
-      acc[2] = mfma<OP>(ar, bi, acc[2]);
-      acc[3] = mfma<OP>(ai, br, acc[3]);
-      if (SPLIT) {
-        const u32x4_t arl = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][1][row * kPairPitch + pofs]);
-        const u32x4_t ail = *reinterpret_cast<const u32x4_t*>(&ldsA[buf][3][row * kPairPitch + pofs]);
-        const u32x4_t brl = *reinterpret_cast<const u32x4_t*>(bt + 1 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
-        const u32x4_t bil = *reinterpret_cast<const u32x4_t*>(bt + 3 * kPairTileY * kPairPitch + col * kPairPitch + pofs);
-        acc[0] = mfma<OP>(ar, brl, acc[0]);
-        acc[1] = mfma<OP>(ai, bil, acc[1]);
-        acc[2] = mfma<OP>(ar, bil, acc[2]);
-        acc[3] = mfma<OP>(ai, brl, acc[3]);
-        acc[0] = mfma<OP>(arl, br, acc[0]);
-        acc[1] = mfma<OP>(ail, bi, acc[1]);
-        acc[2] = mfma<OP>(arl, bi, acc[2]);
-        acc[3] = mfma<OP>(ail, br, acc[3]);
-      }
-    };
 
     const int y = y0 + wy * 32 + (lane & 31);
     const float fv = sh.sy * static_cast<float>(y);
-    const float fv0 = sh.sy * static_cast<float>(static_cast<int>(y_first));
-    // S(+-x) += W_b Phi(r h_b,node) S_node(+-x); acc = 0.  Every lane also folds the x = 0
-    // column sums of both nodes (its generation duty).
+    const int xl = xa0 + wx * 32 + (lane >> 5) * 4;  // |x| of output i is xl + (i >> 2) * 8 + (i & 3)
+    // S(+-x) += W_b sum_t Phi(r h_bt) S_t(+-x) of the accumulators; acc = 0.
     auto fold = [&](int b) {
       const float* nd = job.nodes + b * kMaxNodes;
       const float ws = job.bininfo[b].z;
-      const float hn = nd[node];
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int xa = xa0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
-        const float fu = sh.sx * static_cast<float>(xa);
-        const float f = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * hn) * ws;
-        const float prr = acc[0][i], pii = acc[1][i], pri = acc[2][i], pir = acc[3][i];
-        sp[0][i] = fmaf(f, prr - pii, sp[0][i]);
-        sp[1][i] = fmaf(f, pri + pir, sp[1][i]);
-        sm[0][i] = fmaf(f, prr + pii, sm[0][i]);
-        sm[1][i] = fmaf(f, pri - pir, sm[1][i]);
-        __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
-      }
-      acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
-      if (zero_col) {
+      for (int t = 0; t < T; ++t) {
+        const float hn = nd[t];
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const float f0 = phi<BASIS, MODE>(fabsf(fv0) * nd[t]) * ws;
-          z0r = fmaf(f0, zr[t], z0r);
-          z0i = fmaf(f0, zi[t], z0i);
-          zr[t] = zi[t] = 0.0f;
+        for (int i = 0; i < 16; ++i) {
+          const float fu = sh.sx * static_cast<float>(xl + (i >> 2) * 8 + (i & 3));
+          const float f = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * hn) * ws;
+          const float prr = acc[t][0][i], pii = acc[t][1][i], pri = acc[t][2][i], pir = acc[t][3][i];
+          sp[0][i] = fmaf(f, prr - pii, sp[0][i]);
+          sp[1][i] = fmaf(f, pri + pir, sp[1][i]);
+          sm[0][i] = fmaf(f, prr + pii, sm[0][i]);
+          sm[1][i] = fmaf(f, pri - pir, sm[1][i]);
+          __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
         }
+        acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = (f32x16_t){};
       }
     };
 
-    // Packets, Lagrange weights and step bins are loaded one step ahead of their
-    // use, so their latency hides behind a whole step.  The x = 0 contributions of a
-    // step are generated with it and added to the column sums when the step is
-    // multiplied, i.e. after the fold of the previous bin.
-    Ahead an{};
-    float pr[T] = {}, pi[T] = {};  // x = 0 contributions of the step generated last
-    int cur = 0, bn = 0;
-    if (item.s0 < item.s1) {
-      Ahead a0;
-      fetch(item.s0, &a0);
-      cur = step_bin(item.s0);
-      if (item.s0 + 1 < item.s1) {
-        fetch(item.s0 + 1, &an);
-        bn = step_bin(item.s0 + 1);
-      }
-      generate(a0, 0, pr, pi);
-    }
     __syncthreads();
-    int since = 0;
-    int b = cur;
-    for (int s = item.s0; s < item.s1; ++s) {
-      const int buf = (s - item.s0) & 1;
-      const int b_next = bn;
-      float cr[T], ci[T];  // step s
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        cr[t] = pr[t];
-        ci[t] = pi[t];
-      }
-      if (s + 1 < item.s1) {
-        const Ahead a = an;
-        if (s + 2 < item.s1) {
-          fetch(s + 2, &an);
-          bn = step_bin(s + 2);
-        }
-        generate(a, buf ^ 1, pr, pi);
-      }
-      if (b != cur) {
+    int cur = n_stages > 0 ? stage_bin(0) : 0, since = 0;
+    for (int k = 0; k < n_stages; ++k) {
+      const int b = stage_bin(k);
+      if (b != cur || since >= kBFlushSteps) {
         fold(cur);
         cur = b;
         since = 0;
       }
-      if (zero_col) {
+      const int buf = k & 1;
+      const uint32_t* A = ldsA(buf, 0);
+      // The lane's LDS offsets are recomputed every stage from the lane id (a few VALU
+      // instructions) rather than kept live across the fold, where they would spill.
+      int lid = static_cast<int>(__lane_id());
+      asm volatile("" : "+v"(lid));
+#pragma unroll
+      for (int kk = 0; kk < kPairStage / 16; ++kk) {
+        const int pofs = kk * 8 + (lid >> 5) * 4;
+        const int ra = (wx * 32 + (lid & 31)) * kPairPitch + pofs;
+        const int rb = (wy * 32 + (lid & 31)) * kPairPitch + pofs;
+        const u32x4_t ar = *reinterpret_cast<const u32x4_t*>(A + 0 * kA + ra);
+        const u32x4_t ai = *reinterpret_cast<const u32x4_t*>(A + 2 * kA + ra);
+        u32x4_t arl, ail;
+        if (SPLIT) {
+          arl = *reinterpret_cast<const u32x4_t*>(A + 1 * kA + ra);
+          ail = *reinterpret_cast<const u32x4_t*>(A + 3 * kA + ra);
+        }
 #pragma unroll
         for (int t = 0; t < T; ++t) {
-          zr[t] += cr[t];
-          zi[t] += ci[t];
+          const uint32_t* B = ldsB(buf, t, 0);
+          const u32x4_t br = *reinterpret_cast<const u32x4_t*>(B + 0 * kB + rb);
+          const u32x4_t bi = *reinterpret_cast<const u32x4_t*>(B + 2 * kB + rb);
+          acc[t][0] = mfma<OP>(ar, br, acc[t][0]);
+          acc[t][1] = mfma<OP>(ai, bi, acc[t][1]);
+          acc[t][2] = mfma<OP>(ar, bi, acc[t][2]);
+          acc[t][3] = mfma<OP>(ai, br, acc[t][3]);
+          if (SPLIT) {
+            const u32x4_t brl = *reinterpret_cast<const u32x4_t*>(B + 1 * kB + rb);
+            const u32x4_t bil = *reinterpret_cast<const u32x4_t*>(B + 3 * kB + rb);
+            acc[t][0] = mfma<OP>(ar, brl, acc[t][0]);
+            acc[t][1] = mfma<OP>(ai, bil, acc[t][1]);
+            acc[t][2] = mfma<OP>(ar, bil, acc[t][2]);
+            acc[t][3] = mfma<OP>(ai, brl, acc[t][3]);
+            acc[t][0] = mfma<OP>(arl, br, acc[t][0]);
+            acc[t][1] = mfma<OP>(ail, bi, acc[t][1]);
+            acc[t][2] = mfma<OP>(arl, bi, acc[t][2]);
+            acc[t][3] = mfma<OP>(ail, br, acc[t][3]);
+          }
+          __builtin_amdgcn_sched_barrier(0);  // one node's operands at a time (register budget)
         }
       }
-      consume(buf);
-      if (++since == kBFlushSteps) {
-        fold(cur);
-        since = 0;
-      }
-      b = b_next;
+      since += 2;
       __syncthreads();
     }
-    if (since) fold(cur);
-
-    // Node 1 hands its sums to node 0.
-    if (node == 1) {
-      float* xw = xchg + (wave & 3) * (4 * 16 * 64);
+    if (n_stages > 0) fold(cur);
+    // The sums into chunk slot 2 * chunk, zeros into slot 2 * chunk + 1 (every item writes its
+    // whole tiles, also without particles).
+    if (y < sh.cy) {
+      float2* out0 = job.partial + static_cast<size_t>(2 * item.chunk) * two_cx * sh.cy + y;
+      float2* out1 = out0 + static_cast<size_t>(two_cx) * sh.cy;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        xw[(0 * 16 + i) * 64 + lane] = sp[0][i];
-        xw[(1 * 16 + i) * 64 + lane] = sp[1][i];
-        xw[(2 * 16 + i) * 64 + lane] = sm[0][i];
-        xw[(3 * 16 + i) * 64 + lane] = sm[1][i];
-      }
-    }
-    __syncthreads();
-    if (node == 0 && y < sh.cy) {
-      const float* xw = xchg + (wave & 3) * (4 * 16 * 64);
-      float2* out = job.partial + static_cast<size_t>(item.chunk) * two_cx * sh.cy + y;
-#pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int xa = xa0 + wx * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const int xa = xl + (i >> 2) * 8 + (i & 3);
         if (xa > sh.cx) continue;
-        out[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] =
-            make_float2(sp[0][i] + xw[(0 * 16 + i) * 64 + lane], sp[1][i] + xw[(1 * 16 + i) * 64 + lane]);
-        if (xa < sh.cx)
-          out[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] =
-              make_float2(sm[0][i] + xw[(2 * 16 + i) * 64 + lane], sm[1][i] + xw[(3 * 16 + i) * 64 + lane]);
+        out0[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(sp[0][i], sp[1][i]);
+        out1[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(0.0f, 0.0f);
+        if (xa < sh.cx) {
+          out0[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(sm[0][i], sm[1][i]);
+          out1[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(0.0f, 0.0f);
+        }
       }
     }
-    if (zero_col) {
-#pragma unroll
-      for (int d = 1; d < 8; d <<= 1) {
-        z0r += __shfl_xor(z0r, d);
-        z0i += __shfl_xor(z0i, d);
-      }
-      if (pp == 0 && static_cast<int>(y_first) < sh.cy)
-        job.partial[(static_cast<size_t>(item.chunk) * two_cx + (sh.cx - 1)) * sh.cy + y_first] = make_float2(z0r, z0i);
-    }
+    __syncthreads();  // with the producers' x = 0 read-out
   }
 }
 
@@ -569,9 +625,12 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
     binfo[b].z = std::max(binfo[b].z, std::fabs(cons[i].weight));
   }
   for (int b = 0; b < nb; ++b) binfo[b].w = binfo[b].z > 0.0f ? 1.0f / binfo[b].z : 0.0f;
-  std::vector<int> start(nb + 1, 0);  // in slots, each bin padded to kStep
-  for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kStep - 1) / kStep) * kStep;
-  const int n_slots = std::max(start[nb], kStep);
+  // In slots, each bin padded to two steps (32 particles): the two-node kernel's stages
+  // never straddle a bin.
+  constexpr int kBinPad = 2 * kStep;
+  std::vector<int> start(nb + 1, 0);
+  for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kBinPad - 1) / kBinPad) * kBinPad;
+  const int n_slots = std::max(start[nb], kBinPad);
   std::vector<int> map(n_slots, -1);
   std::vector<int> fill(start.begin(), start.end() - 1);
   for (int i = 0; i < n; ++i) map[fill[bin_of(h[i])]++] = i;
@@ -663,7 +722,7 @@ static void launch_bt(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaIt
 template <int OP, bool SPLIT, int BASIS, int MODE>
 static void launch_pair(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
                         int* counter) {
-  hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE>), dim3(grid), dim3(kThreads), 0, s, jobs, items,
+  hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE>), dim3(grid), dim3(kPairThreads), 0, s, jobs, items,
                      sh, counter);
 }
 
