@@ -204,6 +204,39 @@ __global__ __launch_bounds__(256) void finalize_spectrum_kernel(const double2* _
   F[i] = make_float2(static_cast<float>(a.x * sc), static_cast<float>(-a.y * sc));
 }
 
+// Batched reduction + finalize: block row r = render; partial index f = j * cy + y (j = x + cx - 1)
+// is stored at F[y * 2cx + x0], x0 the reference column of x.
+__global__ __launch_bounds__(256) void batch_finish_kernel(const RenderOut* __restrict__ r,
+                                                           const JobPart* __restrict__ jp, int cx, int cy) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int two_cx = 2 * cx;
+  const int nfreq = two_cx * cy;
+  if (f >= nfreq) return;
+  const RenderOut ro = r[blockIdx.y];
+  double re = 0.0, im = 0.0;
+  for (int j = ro.job_begin; j < ro.job_end; ++j) {
+    const JobPart p = jp[j];
+    for (int k = 0; k < p.nchunks; ++k) {
+      const float2 v = p.partial[static_cast<size_t>(k) * nfreq + f];
+      re += static_cast<double>(v.x);
+      im += static_cast<double>(v.y);
+    }
+  }
+  const int j = f / cy;
+  const int y = f - j * cy;
+  const int x = j - cx + 1;
+  const int x0 = (x >= 0) ? x : x + two_cx;
+  const double sc = static_cast<double>(ro.scale);
+  ro.F[y * two_cx + x0] = make_float2(static_cast<float>(re * sc), static_cast<float>(-im * sc));
+}
+
+hipError_t launch_batch_finish(const RenderOut* r, const JobPart* jp, int n_renders, int cx, int cy, hipStream_t s) {
+  const int nfreq = 2 * cx * cy;
+  if (nfreq <= 0 || n_renders <= 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_finish_kernel, dim3((nfreq + 255) / 256, n_renders), dim3(256), 0, s, r, jp, cx, cy);
+  return hipGetLastError();
+}
+
 hipError_t launch_finalize_spectrum(const double2* acc, int nfreq, float scale, float2* F, hipStream_t s) {
   if (nfreq <= 0) return hipSuccess;
   hipLaunchKernelGGL(finalize_spectrum_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, acc, nfreq, scale, F);

@@ -78,14 +78,13 @@ __device__ float2* lds_inverse_fft(float2* src, float2* dst, int lines, int N, i
   return dst;
 }
 
-__global__ __launch_bounds__(kFftThreads) void c2r_cols_kernel(const float2* __restrict__ F, int cx, int cy, int Nx,
-                                                              int log2Nx, int Ny, int lines,
-                                                              const float2* __restrict__ tw,
-                                                              float2* __restrict__ work) {
+// Lines y0.. of the spectrum through the first pass (x) into work[r][y].
+__device__ __forceinline__ void c2r_cols_body(const float2* __restrict__ F, int cx, int cy, int Nx, int log2Nx, int Ny,
+                                              int lines, int y0, const float2* __restrict__ tw,
+                                              float2* __restrict__ work) {
   __shared__ float2 buf[2 * kFftLdsComplex];
   float2* b0 = buf;
   float2* b1 = buf + kFftLdsComplex;
-  const int y0 = blockIdx.x * lines;
   const int nl = min(lines, cy - y0);
   const int two_cx = 2 * cx;
   const int tid = threadIdx.x;
@@ -108,13 +107,13 @@ __global__ __launch_bounds__(kFftThreads) void c2r_cols_kernel(const float2* __r
   }
 }
 
-__global__ __launch_bounds__(kFftThreads) void c2r_rows_kernel(const float2* __restrict__ work, int cy, int Nx, int Ny,
-                                                              int log2Ny, int rows, const float2* __restrict__ tw,
-                                                              float* __restrict__ image) {
+// Rows r0.. of work through the second pass (y, Hermitian completion) into the image.
+__device__ __forceinline__ void c2r_rows_body(const float2* __restrict__ work, int cy, int Nx, int Ny, int log2Ny,
+                                              int rows, int r0, const float2* __restrict__ tw,
+                                              float* __restrict__ image) {
   __shared__ float2 buf[2 * kFftLdsComplex];
   float2* b0 = buf;
   float2* b1 = buf + kFftLdsComplex;
-  const int r0 = blockIdx.x * rows;
   const int nr = min(rows, Nx - r0);
   const int ld = Ny / 2 + 1;
   const int tid = threadIdx.x;
@@ -138,6 +137,34 @@ __global__ __launch_bounds__(kFftThreads) void c2r_rows_kernel(const float2* __r
   }
 }
 
+__global__ __launch_bounds__(kFftThreads) void c2r_cols_kernel(const float2* __restrict__ F, int cx, int cy, int Nx,
+                                                              int log2Nx, int Ny, int lines,
+                                                              const float2* __restrict__ tw,
+                                                              float2* __restrict__ work) {
+  c2r_cols_body(F, cx, cy, Nx, log2Nx, Ny, lines, blockIdx.x * lines, tw, work);
+}
+
+__global__ __launch_bounds__(kFftThreads) void c2r_rows_kernel(const float2* __restrict__ work, int cy, int Nx, int Ny,
+                                                              int log2Ny, int rows, const float2* __restrict__ tw,
+                                                              float* __restrict__ image) {
+  c2r_rows_body(work, cy, Nx, Ny, log2Ny, rows, blockIdx.x * rows, tw, image);
+}
+
+// Batched: block row blockIdx.y = render.
+__global__ __launch_bounds__(kFftThreads) void c2r_cols_batch_kernel(const RenderOut* __restrict__ r, int cx, int cy,
+                                                                    int Nx, int log2Nx, int Ny, int lines,
+                                                                    const float2* __restrict__ tw) {
+  const RenderOut ro = r[blockIdx.y];
+  c2r_cols_body(ro.F, cx, cy, Nx, log2Nx, Ny, lines, blockIdx.x * lines, tw, ro.work);
+}
+
+__global__ __launch_bounds__(kFftThreads) void c2r_rows_batch_kernel(const RenderOut* __restrict__ r, int cy, int Nx,
+                                                                    int Ny, int log2Ny, int rows,
+                                                                    const float2* __restrict__ tw) {
+  const RenderOut ro = r[blockIdx.y];
+  c2r_rows_body(ro.work, cy, Nx, Ny, log2Ny, rows, blockIdx.x * rows, tw, ro.image);
+}
+
 static int ilog2_pow2(int n) {
   if (n <= 0 || (n & (n - 1))) return -1;
   int l = 0;
@@ -156,6 +183,21 @@ hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const flo
   const int rows = max(1, min(Nx, kFftLdsComplex / Ny));
   hipLaunchKernelGGL(c2r_rows_kernel, dim3((Nx + rows - 1) / rows), dim3(kFftThreads), 0, s, work, cy, Nx, Ny,
                      ilog2_pow2(Ny), rows, tw_y, image);
+  return hipGetLastError();
+}
+
+hipError_t launch_c2r_batch(const RenderOut* r, int n_renders, int cx, int cy, int Nx, int Ny, const float2* tw_x,
+                            const float2* tw_y, hipStream_t s) {
+  if (Nx > kFftLdsComplex || Ny > kFftLdsComplex) return hipErrorInvalidValue;
+  if (n_renders <= 0) return hipSuccess;
+  if (cy > 0) {
+    const int lines = max(1, min(cy, kFftLdsComplex / Nx));
+    hipLaunchKernelGGL(c2r_cols_batch_kernel, dim3((cy + lines - 1) / lines, n_renders), dim3(kFftThreads), 0, s, r, cx,
+                       cy, Nx, ilog2_pow2(Nx), Ny, lines, tw_x);
+  }
+  const int rows = max(1, min(Nx, kFftLdsComplex / Ny));
+  hipLaunchKernelGGL(c2r_rows_batch_kernel, dim3((Nx + rows - 1) / rows, n_renders), dim3(kFftThreads), 0, s, r, cy,
+                     Nx, Ny, ilog2_pow2(Ny), rows, tw_y);
   return hipGetLastError();
 }
 

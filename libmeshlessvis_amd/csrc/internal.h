@@ -79,6 +79,41 @@ hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const flo
                                       int n_slots, double sx, double sy, double3 u, double3 v, Packet* out,
                                       hipStream_t s);
 
+// One job of a batched packet preparation: the packets of a (view, group) pair.
+struct PrepJob {
+  const Constraint* c;
+  const float* radii;   // nullptr: no radii
+  const int* slot_map;  // nullptr: slot k is particle k (k < n), else slot_map[k] (negative: padding)
+  Packet* out;
+  int n, n_slots;
+  double sx, sy;
+  double3 u, v;
+};
+// Every job's packets in one launch; begin[j] = first flattened slot of job j (prefix sums of
+// n_slots, ascending), total = all slots.
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s);
+
+// One render of a batch after the MFMA launches: its jobs' split-K partials
+// (JobPart[job_begin, job_end), summed in order in FP64) give F = scale * (re, -im);
+// then the C2R through work into image.
+struct JobPart {
+  const float2* partial;  // [nchunks][2cx][cy]
+  int nchunks;
+  int pad;
+};
+struct RenderOut {
+  float2* F;
+  float2* work;
+  float* image;
+  float scale;
+  int job_begin, job_end;
+  int pad;
+};
+hipError_t launch_batch_finish(const RenderOut* r, const JobPart* jp, int n_renders, int cx, int cy, hipStream_t s);
+// Batched C2R of n_renders spectra of one grid (cx, cy, Nx, Ny).
+hipError_t launch_c2r_batch(const RenderOut* r, int n_renders, int cx, int cy, int Nx, int Ny, const float2* tw_x,
+                            const float2* tw_y, hipStream_t s);
+
 // direct_sum.hip: plain FP32 direct sum over [first, first+count) packets.
 //   phi_kind: 0 = uniform radius (Phi hoisted per frequency), 1 = per-particle radius.
 //   partial: [ksplit][2cx*cy] double2, slot k holds the sum of split k (unscaled, +i sin).

@@ -63,8 +63,14 @@ struct MfmaWorkspace {
 bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, MfmaBatch* bt);
 // Launches a prepared batch: packet prep, persistent MFMA kernels, reductions (graph-capturable).
 bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev);
+// Only the work-queue reset and the persistent MFMA kernels (packets prepared, partials
+// reduced by the caller).
+bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev);
 // prepare + enqueue.
 bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
+
+// Slot -> particle map of a planned job (nullptr: slot i holds particle i, zeros past n).
+const int* mfma_slot_map(const MfmaPlanned& pl);
 
 // Which kernel variant a group takes (MfmaKind).
 int mfma_kind(const GroupInfo& info, int basis);

@@ -558,10 +558,11 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   return true;
 }
 
+const int* mfma_slot_map(const MfmaPlanned& pl) { return pl.bc ? pl.bc->slot_map.as<int>() : nullptr; }
+
 // Device half: kernel launches only (safe to capture into a HIP graph).
 bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
   if (specs.empty()) return true;
-  const VisConfig* c0 = specs[0].c;
   for (size_t k = 0; k < specs.size(); ++k) {
     const MfmaJobSpec& sp = specs[k];
     const MfmaPlanned& pl = bt.plans[k];
@@ -575,6 +576,19 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
                               sp.c->step_size.y, sp.u, sp.v, const_cast<Packet*>(pl.job.packets), s);
     if (!hip_ok(e, "prep_packets launch")) return false;
   }
+  if (!mfma_launch_sums(specs, bt, s, ev)) return false;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
+    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, bt.plans[k].job.partial,
+                       bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
+    if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
+  }
+  return true;
+}
+
+bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
+  if (specs.empty()) return true;
+  const VisConfig* c0 = specs[0].c;
   hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, bt.counters, static_cast<int>(bt.keys.size()));
   if (!hip_ok(hipGetLastError(), "zero counters")) return false;
 
@@ -607,13 +621,6 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
     if (!hip_ok(hipGetLastError(), "mfma launch")) return false;
   }
   if (ev && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
-
-  for (size_t k = 0; k < specs.size(); ++k) {
-    const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
-    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, bt.plans[k].job.partial,
-                       bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
-    if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
-  }
   return true;
 }
 

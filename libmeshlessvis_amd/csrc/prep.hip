@@ -19,40 +19,9 @@ __device__ __forceinline__ uint32_t to_fixed_rev(double a) {
   return static_cast<uint32_t>(static_cast<uint64_t>(scaled));  // 2^32 wraps to 0
 }
 
-__global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __restrict__ c,
-                                                           const float* __restrict__ radii, int n, int n_padded,
-                                                           double sx, double sy, double3 u, double3 v,
-                                                           Packet* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_padded) return;
-  Packet p;
-  if (i < n) {
-    const float4 q = reinterpret_cast<const float4*>(c)[i];
-    const double px = q.x, py = q.y, pz = q.z;
-    const double du = fma(u.x, px, fma(u.y, py, u.z * pz));
-    const double dv = fma(v.x, px, fma(v.y, py, v.z * pz));
-    p.pu = to_fixed_rev(sx * du);
-    p.pv = to_fixed_rev(sy * dv);
-    p.w = q.w;
-    p.h = radii ? radii[i] : 1.0f;
-  } else {
-    p.pu = 0u;
-    p.pv = 0u;
-    p.w = 0.0f;
-    p.h = 0.0f;
-  }
-  out[i] = p;
-}
-
-// Same packets in a given slot order: slot k takes particle slot_map[k] (zero packet when negative).
-__global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constraint* __restrict__ c,
-                                                                  const float* __restrict__ radii,
-                                                                  const int* __restrict__ slot_map, int n_slots,
-                                                                  double sx, double sy, double3 u, double3 v,
-                                                                  Packet* __restrict__ out) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_slots) return;
-  const int i = slot_map[k];
+// The packet of particle i (a zero packet for i < 0).
+__device__ __forceinline__ Packet make_packet(const Constraint* __restrict__ c, const float* __restrict__ radii, int i,
+                                              double sx, double sy, double3 u, double3 v) {
   Packet p;
   if (i >= 0) {
     const float4 q = reinterpret_cast<const float4*>(c)[i];
@@ -69,7 +38,55 @@ __global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constrai
     p.w = 0.0f;
     p.h = 0.0f;
   }
-  out[k] = p;
+  return p;
+}
+
+__global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __restrict__ c,
+                                                           const float* __restrict__ radii, int n, int n_padded,
+                                                           double sx, double sy, double3 u, double3 v,
+                                                           Packet* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_padded) return;
+  out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v);
+}
+
+// Same packets in a given slot order: slot k takes particle slot_map[k] (zero packet when negative).
+__global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constraint* __restrict__ c,
+                                                                  const float* __restrict__ radii,
+                                                                  const int* __restrict__ slot_map, int n_slots,
+                                                                  double sx, double sy, double3 u, double3 v,
+                                                                  Packet* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n_slots) return;
+  out[k] = make_packet(c, radii, slot_map[k], sx, sy, u, v);
+}
+
+// All jobs of a batch: thread g of the flattened slot range finds its job by binary search.
+__global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restrict__ jobs,
+                                                        const long long* __restrict__ begin, int n_jobs,
+                                                        long long total) {
+  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (begin[mid] <= g)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const PrepJob& J = jobs[lo];
+  const int k = static_cast<int>(g - begin[lo]);
+  if (k >= J.n_slots) return;
+  const int i = J.slot_map ? J.slot_map[k] : (k < J.n ? k : -1);
+  J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v);
+}
+
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s) {
+  if (n_jobs <= 0 || total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_jobs_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, jobs, begin,
+                     n_jobs, total);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,

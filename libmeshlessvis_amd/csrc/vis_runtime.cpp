@@ -90,6 +90,7 @@ struct BatchGraph {
   hipGraphExec_t exec = nullptr;
   hipStream_t capture = nullptr;  // default priority
   mv::MfmaWorkspace ws;
+  mv::DevBuf io;  // batched prep / finish / C2R tables
   void drop() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -100,6 +101,7 @@ struct BatchGraph {
   void release() {
     drop();
     ws.release();
+    io.release();
     if (capture) (void)hipStreamDestroy(capture);
     capture = nullptr;
   }
@@ -421,10 +423,68 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
   return k;
 }
 
+// Device tables of the batched prep / finish / C2R launches (uploaded before capture).
+static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, VisConfig** cf, int n,
+                             const std::vector<int>& render_of, BatchGraph* bg, const PrepJob** jobs,
+                             const long long** begin, const RenderOut** outs, const JobPart** parts,
+                             long long* total) {
+  const size_t nj = specs.size();
+  std::vector<PrepJob> pj(nj);
+  std::vector<long long> bg_begin(nj);
+  std::vector<JobPart> jp(nj);
+  std::vector<RenderOut> ro(n);
+  long long acc = 0;
+  for (size_t k = 0; k < nj; ++k) {
+    const MfmaJobSpec& sp = specs[k];
+    const MfmaPlanned& pl = bt.plans[k];
+    PrepJob& J = pj[k];
+    J.c = sp.g->d_constraints;
+    J.radii = sp.g->d_radii;
+    J.slot_map = mfma_slot_map(pl);
+    J.out = const_cast<Packet*>(pl.job.packets);
+    J.n = sp.n;
+    J.n_slots = pl.n_slots;
+    J.sx = sp.c->step_size.x;
+    J.sy = sp.c->step_size.y;
+    J.u = sp.u;
+    J.v = sp.v;
+    bg_begin[k] = acc;
+    acc += pl.n_slots;
+    jp[k].partial = pl.job.partial;
+    jp[k].nchunks = pl.nchunks;
+  }
+  for (int i = 0; i < n; ++i) {
+    ro[i] = RenderOut{cf[i]->_d_freq_image, cf[i]->_d_freq_image_arranged, cf[i]->_d_image, cf[i]->_scale, 0, 0, 0};
+    ro[i].job_begin = static_cast<int>(std::find(render_of.begin(), render_of.end(), i) - render_of.begin());
+    ro[i].job_end = ro[i].job_begin;
+    while (ro[i].job_end < static_cast<int>(nj) && render_of[ro[i].job_end] == i) ++ro[i].job_end;
+  }
+  const size_t b_pj = sizeof(PrepJob) * nj, b_bg = sizeof(long long) * nj, b_ro = sizeof(RenderOut) * n,
+               b_jp = sizeof(JobPart) * nj;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t o_bg = up(b_pj), o_ro = o_bg + up(b_bg), o_jp = o_ro + up(b_ro), all = o_jp + up(b_jp);
+  if (!bg->io.reserve(all)) return false;
+  std::vector<char> h(all, 0);
+  std::memcpy(h.data(), pj.data(), b_pj);
+  std::memcpy(h.data() + o_bg, bg_begin.data(), b_bg);
+  std::memcpy(h.data() + o_ro, ro.data(), b_ro);
+  std::memcpy(h.data() + o_jp, jp.data(), b_jp);
+  if (!hip_ok(hipMemcpy(bg->io.ptr, h.data(), all, hipMemcpyHostToDevice), "upload batch tables")) return false;
+  char* base = static_cast<char*>(bg->io.ptr);
+  *jobs = reinterpret_cast<const PrepJob*>(base);
+  *begin = reinterpret_cast<const long long*>(base + o_bg);
+  *outs = reinterpret_cast<const RenderOut*>(base + o_ro);
+  *parts = reinterpret_cast<const JobPart*>(base + o_jp);
+  *total = acc;
+  return true;
+}
+
 static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGraph* bg) {
   bg->drop();
   hipStream_t s = cf[0]->_plan->stream;
   std::vector<MfmaJobSpec> specs, one;
+  std::vector<int> render_of;  // render of every job
+  bool same_grid = true;       // every render has the samples of render 0: batched C2R
   for (int i = 0; i < n; ++i) {
     VisConfig* c = cf[i];
     std::string why;
@@ -440,22 +500,48 @@ static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
     if (!c->_plan->acc.reserve(sizeof(double2) * std::max<size_t>(spectrum_count(c), 1))) return false;
     if (!collect_groups(ds[i], c, c->_plan->acc.as<double2>(), &one)) return false;
     specs.insert(specs.end(), one.begin(), one.end());
+    render_of.insert(render_of.end(), one.size(), i);
+    same_grid = same_grid && c->_number_of_samples.x == cf[0]->_number_of_samples.x &&
+                c->_number_of_samples.y == cf[0]->_number_of_samples.y;
   }
+  same_grid = same_grid && n <= 65535;  // render = grid row of the batched launches
   if (!mfma_prepare(specs, s, &bg->ws, &bg->ws.batch)) return false;
+  const PrepJob* pjobs = nullptr;
+  const long long* pbegin = nullptr;
+  const RenderOut* outs = nullptr;
+  const JobPart* parts = nullptr;
+  long long total = 0;
+  if (same_grid &&
+      !upload_io_tables(specs, bg->ws.batch, cf, n, render_of, bg, &pjobs, &pbegin, &outs, &parts, &total))
+    return false;
   if (!bg->capture && !hip_ok(hipStreamCreateWithFlags(&bg->capture, hipStreamNonBlocking), "hipStreamCreate"))
     return false;
   if (!hip_ok(hipStreamBeginCapture(bg->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
     return false;
-  bool ok = mfma_enqueue(specs, bg->ws.batch, bg->capture, nullptr);
-  for (int i = 0; ok && i < n; ++i) {
-    VisConfig* c = cf[i];
-    const int nfreq = static_cast<int>(spectrum_count(c));
-    ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, bg->capture),
-                "finalize launch") &&
-         hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                           c->_number_of_samples.y, c->_plan->tw_x, c->_plan->tw_y, c->_d_freq_image_arranged,
-                           c->_d_image, bg->capture),
+  bool ok;
+  if (same_grid) {
+    // One launch each: every job's packets, the persistent MFMA kernels, every render's
+    // reduction + finalize, and the two C2R passes of every render.
+    const VisConfig* c0 = cf[0];
+    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, bg->capture), "prep launch") &&
+         mfma_launch_sums(specs, bg->ws.batch, bg->capture, nullptr) &&
+         hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, bg->capture),
+                "finish launch") &&
+         hip_ok(launch_c2r_batch(outs, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, c0->_number_of_samples.x,
+                                 c0->_number_of_samples.y, c0->_plan->tw_x, c0->_plan->tw_y, bg->capture),
                 "c2r launch");
+  } else {
+    ok = mfma_enqueue(specs, bg->ws.batch, bg->capture, nullptr);
+    for (int i = 0; ok && i < n; ++i) {
+      VisConfig* c = cf[i];
+      const int nfreq = static_cast<int>(spectrum_count(c));
+      ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, bg->capture),
+                  "finalize launch") &&
+           hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
+                             c->_number_of_samples.y, c->_plan->tw_x, c->_plan->tw_y, c->_d_freq_image_arranged,
+                             c->_d_image, bg->capture),
+                  "c2r launch");
+    }
   }
   hipGraph_t graph = nullptr;
   const hipError_t end = hipStreamEndCapture(bg->capture, &graph);
