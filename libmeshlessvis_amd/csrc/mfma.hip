@@ -108,12 +108,18 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
       float zr[8], zi[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) zr[m] = zi[m] = 0.0f;
+      // Loads without a branch, clamped into the item: a conditional load would make the
+      // compiler wait for it at the branch's join, right after issuing it.  Slots past p1
+      // are zeroed through their weight in generate.
       auto fetch = [&](int k, Packet* q) {
         const int p = p0 + k * kStage + 2 * pp;
-        q[0] = p < p1 ? load_packet(job.packets + p) : Packet{0u, 0u, 0.0f, 0.0f};
-        q[1] = p + 1 < p1 ? load_packet(job.packets + p + 1) : Packet{0u, 0u, 0.0f, 0.0f};
+        q[0] = GAUSS ? load_packet(job.packets + min(p, p1 - 1)) : load_packet_puw(job.packets + min(p, p1 - 1));
+        q[1] = GAUSS ? load_packet(job.packets + min(p + 1, p1 - 1))
+                     : load_packet_puw(job.packets + min(p + 1, p1 - 1));
       };
-      auto generate = [&](const Packet* q, int buf, auto zero) {
+      auto generate = [&](const Packet* q, int k, auto zero) {
+        const int buf = k & 1;
+        const int left = p1 - (p0 + k * kStage + 2 * pp);  // particles of the item from this lane's first
         float dh[2], wq[2];
         uint32_t PA[2], PB[2], dA[2], dB[2];
 #pragma unroll
@@ -121,7 +127,7 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
           // h_ref is the bin edge that keeps the exponents <= 0 (|alpha|, |beta| <= 1).
           const float d = q[u].h * q[u].h - hh_ref;
           dh[u] = sh.ref_hi ? fminf(d, 0.0f) : fmaxf(d, 0.0f);
-          wq[u] = q[u].w * bi.w;
+          wq[u] = u < left ? q[u].w * bi.w : 0.0f;
           PA[u] = xa_first * q[u].pu + kPhaseRound;
           PB[u] = y_first * q[u].pv + kPhaseRound;
           dA[u] = q[u].pu << 4;
@@ -164,24 +170,23 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
         }
       };
       // Packets are loaded one stage ahead of their use, so the load's latency
-      // hides behind a whole stage of operand generation.
+      // hides behind a whole stage of operand generation.  n_stages + 1 barriers, as
+      // the consumers.
       auto produce = [&](auto zero) {
-        Packet qn[2] = {};
         if (n_stages > 0) {
-          Packet q0[2];
+          Packet q0[2], qn[2];
           fetch(0, q0);
-          if (n_stages > 1) fetch(1, qn);
+          fetch(min(1, n_stages - 1), qn);
           generate(q0, 0, zero);
+          __syncthreads();
+          for (int k = 1; k < n_stages; ++k) {  // stage k while the consumers multiply k - 1
+            const Packet q[2] = {qn[0], qn[1]};
+            fetch(min(k + 1, n_stages - 1), qn);
+            generate(q, k, zero);
+            __syncthreads();
+          }
         }
         __syncthreads();
-        for (int k = 0; k < n_stages; ++k) {
-          if (k + 1 < n_stages) {
-            const Packet q[2] = {qn[0], qn[1]};
-            if (k + 2 < n_stages) fetch(k + 2, qn);
-            generate(q, (k + 1) & 1, zero);
-          }
-          __syncthreads();
-        }
       };
       if (zero_col)
         produce(std::true_type{});

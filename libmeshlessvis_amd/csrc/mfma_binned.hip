@@ -70,26 +70,32 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
     constexpr int LW = lag_words(T);
     struct Ahead {
       Packet q;
-      float4 l[LW];
+      float l[T];
     };
+    // Exactly the words generate reads (see load_packet_puw).
     auto fetch = [&](int step, Ahead* a) {
       const size_t slot = static_cast<size_t>(step) * kStep + gp;
-      a->q = load_packet(job.packets + slot);
-#pragma unroll
-      for (int k = 0; k < LW; ++k) a->l[k] = load_float4(job.lag + slot * LW + k);
+      a->q = load_packet_puw(job.packets + slot);
+      load_floats<T>(reinterpret_cast<const float*>(job.lag + slot * LW), a->l);
     };
     auto step_bin = [&](int step) -> int {
       return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + step);
     };
+    // Bins of 64 steps at a time, one per lane (bin_at(k): bin of step s0 + k, k ascending).
+    const int n_steps = item.s1 - item.s0;
+    int bin_blk = -1, bin_v = 0;
+    auto bin_at = [&](int k) -> int {
+      if ((k >> 6) != bin_blk) {
+        bin_blk = k >> 6;
+        const int j = (k & ~63) + lane;
+        bin_v = j < n_steps ? step_bin(item.s0 + j) : 0;
+      }
+      return __builtin_amdgcn_readlane(bin_v, k & 63);
+    };
 
     auto generate = [&](const Ahead& a, int buf) {
       const Packet& q = a.q;
-      float l[T];
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const float4 v = a.l[t >> 2];
-        l[t] = (t & 3) == 0 ? v.x : (t & 3) == 1 ? v.y : (t & 3) == 2 ? v.z : v.w;
-      }
+      const float* l = a.l;
       uint32_t PA = x_first * q.pu + kPhaseRound;
       const uint32_t dA = q.pu << 5;
 #pragma unroll
@@ -100,11 +106,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
         const uint32_t hi = pack2<OP>(re, im);
         const int o = (gg + 32 * m) * kPitch + gp;
         ldsA[buf][0][o] = hi;
-        if (SPLIT) {
-          float hr, hm;
-          unpack2<OP>(hi, &hr, &hm);
-          ldsA[buf][1][o] = pack2<OP>(re - hr, im - hm);
-        }
+        if (SPLIT) ldsA[buf][1][o] = split_lo<OP>(re, im, hi);
       }
       uint32_t PB = y_first * q.pv + kPhaseRound;
       const uint32_t dB = q.pv << 5;
@@ -119,11 +121,7 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
           const float re = l[t] * re0, im = l[t] * im0;
           const uint32_t hi = pack2<OP>(re, im);
           ldsB[buf][t][0][o] = hi;
-          if (SPLIT) {
-            float hr, hm;
-            unpack2<OP>(hi, &hr, &hm);
-            ldsB[buf][t][1][o] = pack2<OP>(re - hr, im - hm);
-          }
+          if (SPLIT) ldsB[buf][t][1][o] = split_lo<OP>(re, im, hi);
         }
       }
     };
@@ -186,32 +184,26 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
       }
     };
 
-    // Packets, Lagrange weights and step bins are loaded one step ahead of their
-    // use, so their latency hides behind a whole step.
+    // Packets and Lagrange weights are loaded one step ahead of their use, so their
+    // latency hides behind a whole step; the loads take no branch (clamped into the item),
+    // since a conditional load makes the compiler wait for it at the branch's join.
     Ahead an{};
-    int cur = 0, bn = 0;  // bin of the first step; bin of the step after the current one
-    if (item.s0 < item.s1) {
+    int cur = 0;
+    if (n_steps > 0) {
       Ahead a0;
       fetch(item.s0, &a0);
-      cur = step_bin(item.s0);
-      if (item.s0 + 1 < item.s1) {
-        fetch(item.s0 + 1, &an);
-        bn = step_bin(item.s0 + 1);
-      }
+      fetch(item.s0 + min(1, n_steps - 1), &an);
+      cur = bin_at(0);
       generate(a0, 0);
     }
     __syncthreads();
     int since = 0;
-    int b = cur;  // bin of step s
-    for (int s = item.s0; s < item.s1; ++s) {
-      const int buf = (s - item.s0) & 1;
-      const int b_next = bn;
-      if (s + 1 < item.s1) {
+    for (int k = 0; k < n_steps; ++k) {
+      const int buf = k & 1;
+      const int b = bin_at(k);
+      if (k + 1 < n_steps) {
         const Ahead a = an;
-        if (s + 2 < item.s1) {
-          fetch(s + 2, &an);
-          bn = step_bin(s + 2);
-        }
+        fetch(item.s0 + min(k + 2, n_steps - 1), &an);
         generate(a, buf ^ 1);
       }
       if (b != cur) {
@@ -224,7 +216,6 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
         fold(cur);
         since = 0;
       }
-      b = b_next;
       __syncthreads();
     }
     if (since) fold(cur);
@@ -302,6 +293,17 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
     auto stage_bin = [&](int k) -> int {
       return *(const __attribute__((address_space(1))) uint16_t*)(job.step_bin + item.s0 + 2 * k);
     };
+    // Bins of 64 stages at a time, one per lane (a wave-uniform global load per stage would
+    // stall every stage for its latency); bin_at(k) needs k ascending.
+    int bin_blk = -1, bin_v = 0;
+    auto bin_at = [&](int k) -> int {
+      if ((k >> 6) != bin_blk) {
+        bin_blk = k >> 6;
+        const int j = (k & ~63) + lane;
+        bin_v = j < n_stages ? stage_bin(j) : 0;
+      }
+      return __builtin_amdgcn_readlane(bin_v, k & 63);
+    };
 
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
@@ -319,9 +321,8 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
         const size_t slot = static_cast<size_t>(item.s0) * kStep + k * kPairStage + 2 * pp;
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-          a->q[u] = load_packet(job.packets + slot + u);
-          const float4 l = load_float4(job.lag + slot + u);
-          a->l[u] = make_float2(l.x, l.y);
+          a->q[u] = load_packet_puw(job.packets + slot + u);
+          a->l[u] = load_float2(job.lag + slot + u);
         }
       };
       // x = 0: this lane's column sums of B_t per column m (current bin); folded into z0.
@@ -395,38 +396,38 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
           }
         }
       };
-      // Packets and Lagrange weights are loaded one stage ahead of their use.  The x = 0
-      // sums are folded when the stage about to be generated opens a new bin, and every
-      // kBFlushSteps steps, as the consumers fold theirs.
+      // Packets and Lagrange weights are loaded one stage ahead of their use, without a
+      // branch (a conditional load would make the compiler wait for it at the branch's
+      // join, right after issuing it).  The x = 0 sums are folded when the stage about to
+      // be generated opens a new bin, and every kBFlushSteps steps, as the consumers fold
+      // theirs.  n_stages + 1 barriers, as the consumers.
       auto produce = [&](auto zero) {
-        Ahead an{};
         int zb = 0, since = 0;
         if (n_stages > 0) {
-          Ahead a0;
+          Ahead a0, an;
+          if (decltype(zero)::value) zb = bin_at(0);
           fetch(0, &a0);
-          zb = stage_bin(0);
-          if (n_stages > 1) fetch(1, &an);
+          fetch(min(1, n_stages - 1), &an);
           generate(a0, 0, zero);
           since = 2;
-        }
-        __syncthreads();
-        for (int k = 0; k < n_stages; ++k) {
-          if (k + 1 < n_stages) {
-            const Ahead a = an;
-            if (k + 2 < n_stages) fetch(k + 2, &an);
+          __syncthreads();
+          for (int k = 1; k < n_stages; ++k) {  // stage k while the consumers multiply k - 1
             if (decltype(zero)::value) {
-              const int nb = stage_bin(k + 1);
+              const int nb = bin_at(k);
               if (nb != zb || since >= kBFlushSteps) {
                 zfold(zb);
                 zb = nb;
                 since = 0;
               }
             }
-            generate(a, (k + 1) & 1, zero);
+            const Ahead a = an;
+            fetch(min(k + 1, n_stages - 1), &an);
+            generate(a, k & 1, zero);
             since += 2;
+            __syncthreads();
           }
-          __syncthreads();
         }
+        __syncthreads();
         if (decltype(zero)::value && n_stages > 0) zfold(zb);
       };
       if (zero_col)
@@ -486,9 +487,9 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
     };
 
     __syncthreads();
-    int cur = n_stages > 0 ? stage_bin(0) : 0, since = 0;
+    int cur = n_stages > 0 ? bin_at(0) : 0, since = 0;
     for (int k = 0; k < n_stages; ++k) {
-      const int b = stage_bin(k);
+      const int b = bin_at(k);
       if (b != cur || since >= kBFlushSteps) {
         fold(cur);
         cur = b;
@@ -500,40 +501,70 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
       // instructions) rather than kept live across the fold, where they would spill.
       int lid = static_cast<int>(__lane_id());
       asm volatile("" : "+v"(lid));
+      const int rowa = (wx * 32 + (lid & 31)) * kPairPitch + (lid >> 5) * 4;
+      const int rowb = (wy * 32 + (lid & 31)) * kPairPitch + (lid >> 5) * 4;
+      auto lda = [&](int kk, int part) { return *reinterpret_cast<const u32x4_t*>(A + part * kA + rowa + kk * 8); };
+      auto ldb = [&](int kk, int t, int part) {
+        return *reinterpret_cast<const u32x4_t*>(ldsB(buf, t, part) + rowb + kk * 8);
+      };
+      // acc[t] += (x_re, x_im) x (y_re, y_im): the four real products.
+      auto quart = [&](f32x16_t* a, const u32x4_t& xr, const u32x4_t& xi, const u32x4_t& yr, const u32x4_t& yi) {
+        a[0] = mfma<OP>(xr, yr, a[0]);
+        a[1] = mfma<OP>(xi, yi, a[1]);
+        a[2] = mfma<OP>(xr, yi, a[2]);
+        a[3] = mfma<OP>(xi, yr, a[3]);
+      };
+      if (SPLIT) {
+        // Twelve quartets over (kk, node) groups g0 = (0,0), g1 = (0,1), g2 = (1,0), g3 = (1,1),
+        // each split product hi.hi, lo.hi, hi.lo one quartet.  Operands live in four
+        // register pairs - A hi (ah), A lo (al), B hi (bh), B lo (bl) - and every load after
+        // the first four goes into a pair the previous quartet freed, one quartet before
+        // its use, so the LDS latency hides behind MFMAs (the sched_barriers pin the order).
+        u32x4_t ahr = lda(0, 0), ahi = lda(0, 2), bhr = ldb(0, 0, 0), bhi = ldb(0, 0, 2);
+        u32x4_t alr = lda(0, 1), ali = lda(0, 3), blr = ldb(0, 0, 1), bli = ldb(0, 0, 3);
+        auto pin = [] { __builtin_amdgcn_sched_barrier(0); };
+        quart(acc[0], ahr, ahi, bhr, bhi);  // g0 hi.hi
+        pin();
+        quart(acc[0], alr, ali, bhr, bhi);  // g0 lo.hi
+        pin();
+        bhr = ldb(0, 1, 0), bhi = ldb(0, 1, 2);
+        pin();
+        quart(acc[0], ahr, ahi, blr, bli);  // g0 hi.lo
+        pin();
+        blr = ldb(0, 1, 1), bli = ldb(0, 1, 3);
+        pin();
+        quart(acc[1], ahr, ahi, bhr, bhi);  // g1 hi.hi
+        pin();
+        quart(acc[1], ahr, ahi, blr, bli);  // g1 hi.lo
+        pin();
+        ahr = lda(1, 0), ahi = lda(1, 2), blr = ldb(1, 0, 1), bli = ldb(1, 0, 3);
+        pin();
+        quart(acc[1], alr, ali, bhr, bhi);  // g1 lo.hi
+        pin();
+        alr = lda(1, 1), ali = lda(1, 3), bhr = ldb(1, 0, 0), bhi = ldb(1, 0, 2);
+        pin();
+        quart(acc[0], ahr, ahi, blr, bli);  // g2 hi.lo
+        pin();
+        blr = ldb(1, 1, 1), bli = ldb(1, 1, 3);
+        pin();
+        quart(acc[0], ahr, ahi, bhr, bhi);  // g2 hi.hi
+        pin();
+        quart(acc[0], alr, ali, bhr, bhi);  // g2 lo.hi
+        pin();
+        bhr = ldb(1, 1, 0), bhi = ldb(1, 1, 2);
+        pin();
+        quart(acc[1], ahr, ahi, blr, bli);  // g3 hi.lo
+        pin();
+        quart(acc[1], ahr, ahi, bhr, bhi);  // g3 hi.hi
+        pin();
+        quart(acc[1], alr, ali, bhr, bhi);  // g3 lo.hi
+        pin();
+      } else {
 #pragma unroll
-      for (int kk = 0; kk < kPairStage / 16; ++kk) {
-        const int pofs = kk * 8 + (lid >> 5) * 4;
-        const int ra = (wx * 32 + (lid & 31)) * kPairPitch + pofs;
-        const int rb = (wy * 32 + (lid & 31)) * kPairPitch + pofs;
-        const u32x4_t ar = *reinterpret_cast<const u32x4_t*>(A + 0 * kA + ra);
-        const u32x4_t ai = *reinterpret_cast<const u32x4_t*>(A + 2 * kA + ra);
-        u32x4_t arl, ail;
-        if (SPLIT) {
-          arl = *reinterpret_cast<const u32x4_t*>(A + 1 * kA + ra);
-          ail = *reinterpret_cast<const u32x4_t*>(A + 3 * kA + ra);
-        }
+        for (int kk = 0; kk < kPairStage / 16; ++kk) {
+          const u32x4_t ar = lda(kk, 0), ai = lda(kk, 2);
 #pragma unroll
-        for (int t = 0; t < T; ++t) {
-          const uint32_t* B = ldsB(buf, t, 0);
-          const u32x4_t br = *reinterpret_cast<const u32x4_t*>(B + 0 * kB + rb);
-          const u32x4_t bi = *reinterpret_cast<const u32x4_t*>(B + 2 * kB + rb);
-          acc[t][0] = mfma<OP>(ar, br, acc[t][0]);
-          acc[t][1] = mfma<OP>(ai, bi, acc[t][1]);
-          acc[t][2] = mfma<OP>(ar, bi, acc[t][2]);
-          acc[t][3] = mfma<OP>(ai, br, acc[t][3]);
-          if (SPLIT) {
-            const u32x4_t brl = *reinterpret_cast<const u32x4_t*>(B + 1 * kB + rb);
-            const u32x4_t bil = *reinterpret_cast<const u32x4_t*>(B + 3 * kB + rb);
-            acc[t][0] = mfma<OP>(ar, brl, acc[t][0]);
-            acc[t][1] = mfma<OP>(ai, bil, acc[t][1]);
-            acc[t][2] = mfma<OP>(ar, bil, acc[t][2]);
-            acc[t][3] = mfma<OP>(ai, brl, acc[t][3]);
-            acc[t][0] = mfma<OP>(arl, br, acc[t][0]);
-            acc[t][1] = mfma<OP>(ail, bi, acc[t][1]);
-            acc[t][2] = mfma<OP>(arl, bi, acc[t][2]);
-            acc[t][3] = mfma<OP>(ail, br, acc[t][3]);
-          }
-          __builtin_amdgcn_sched_barrier(0);  // one node's operands at a time (register budget)
+          for (int t = 0; t < T; ++t) quart(acc[t], ar, ai, ldb(kk, t, 0), ldb(kk, t, 2));
         }
       }
       since += 2;

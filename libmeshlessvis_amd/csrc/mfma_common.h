@@ -38,10 +38,12 @@ struct MfmaShared {
 };
 
 
+// Two values rounded to nearest even into one word (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32).
 template <int OP>
 __device__ __forceinline__ uint32_t pack2(float a, float b) {
   if (OP == OP_F16) {
-    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(a, b));
+    const _Float16 __attribute__((ext_vector_type(2))) h = {static_cast<_Float16>(a), static_cast<_Float16>(b)};
+    return __builtin_bit_cast(uint32_t, h);
   } else {
     const __bf16 ha = static_cast<__bf16>(a), hb = static_cast<__bf16>(b);
     return static_cast<uint32_t>(__builtin_bit_cast(uint16_t, ha)) |
@@ -80,6 +82,46 @@ __device__ __forceinline__ float4 load_float4(const float4* p) {
   const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t*)p;
   return make_float4(v[0], v[1], v[2], v[3]);
 }
+// Only the fields a kernel reads.  A loaded register the kernel never reads is free to
+// the register allocator while the load is in flight, and reusing it makes the compiler
+// wait for the load right there - in a prefetch, at once.
+typedef uint32_t u32x3_t __attribute__((ext_vector_type(3)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ Packet load_packet_puw(const Packet* p) {  // pu, pv, w (h = 0)
+  const u32x3_t v = *(const __attribute__((address_space(1))) u32x3_t*)p;
+  Packet q;
+  q.pu = v[0];
+  q.pv = v[1];
+  const uint32_t wb = v[2];
+  q.w = __builtin_bit_cast(float, wb);
+  q.h = 0.0f;
+  return q;
+}
+__device__ __forceinline__ float2 load_float2(const float4* p) {  // x, y
+  const f32x2_t v = *(const __attribute__((address_space(1))) f32x2_t*)p;
+  return make_float2(v[0], v[1]);
+}
+// N consecutive floats from a 16-byte aligned address: loads of 4 and one of the rest.
+template <int N>
+__device__ __forceinline__ void load_floats(const float* p, float* out) {
+  typedef const __attribute__((address_space(1))) float* gptr;
+#pragma unroll
+  for (int i = 0; i + 4 <= N; i += 4) {
+    const f32x4_t v = *(const __attribute__((address_space(1))) f32x4_t*)(p + i);
+    out[i] = v[0], out[i + 1] = v[1], out[i + 2] = v[2], out[i + 3] = v[3];
+  }
+  constexpr int r = N & 3, b = N & ~3;
+  if (r == 3) {
+    typedef float f3 __attribute__((ext_vector_type(3)));
+    const f3 v = *(const __attribute__((address_space(1))) f3*)(p + b);
+    out[b] = v[0], out[b + 1] = v[1], out[b + 2] = v[2];
+  } else if (r == 2) {
+    const f32x2_t v = *(const __attribute__((address_space(1))) f32x2_t*)(p + b);
+    out[b] = v[0], out[b + 1] = v[1];
+  } else if (r == 1) {
+    out[b] = *gptr(p + b);
+  }
+}
 
 // Half an LSB of the 23 phase bits phase_rev keeps; added once to a phase so
 // that phase_rev rounds instead of truncating.
@@ -93,6 +135,26 @@ __device__ __forceinline__ float phase_rev(uint32_t P) {
   return __builtin_bit_cast(float, __builtin_amdgcn_alignbit(0x7fu, P, 9));
 }
 
+// The FP16 residuals rnd(a - hi.x), rnd(b - hi.y) of a packed pair hi: v_fma_mix
+// takes the FP16 halves of hi as FP32 operands, forms a - hi exactly (one FMA) and
+// rounds it to FP16 into the low / high half of the result - two instructions in
+// place of two conversions, two subtractions and a pack.
+__device__ __forceinline__ uint32_t split_lo_f16(float a, float b, uint32_t hi) {
+  uint32_t d;
+  asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(hi));
+  asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(d) : "v"(b), "v"(hi));
+  return d;
+}
+
+// rnd(a - hi.x), rnd(b - hi.y): the low words of the split a = hi + lo.
+template <int OP>
+__device__ __forceinline__ uint32_t split_lo(float a, float b, uint32_t hi) {
+  if (OP == OP_F16) return split_lo_f16(a, b, hi);
+  float ha, hb;
+  unpack2<OP>(hi, &ha, &hb);
+  return pack2<OP>(a - ha, b - hb);
+}
+
 // hi = rnd(v), lo = rnd(v - hi) of two consecutive particles' real and imaginary
 // parts, stored as words [re_hi, re_lo, im_hi, im_lo][o] of an LDS array of N words
 // per part (one 16-bit value of each particle per word).
@@ -102,11 +164,8 @@ __device__ __forceinline__ void store_pair(uint32_t (*arr)[N], int o, const floa
   arr[0][o] = hr;
   arr[2][o] = hm;
   if (SPLIT) {
-    float a0, a1, b0, b1;
-    unpack2<OP>(hr, &a0, &a1);
-    unpack2<OP>(hm, &b0, &b1);
-    arr[1][o] = pack2<OP>(re[0] - a0, re[1] - a1);
-    arr[3][o] = pack2<OP>(im[0] - b0, im[1] - b1);
+    arr[1][o] = split_lo<OP>(re[0], re[1], hr);
+    arr[3][o] = split_lo<OP>(im[0], im[1], hm);
   }
 }
 
