@@ -437,7 +437,16 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   const int cap = binned ? 4096 : kMaxChunkSteps;
   const int floor_steps = binned ? 128 : 32;
   out->tile_steps = total_steps * tiles;
-  if (target_steps <= 0) target_steps = total_steps * tiles / 2048;  // one render: ~2048 items
+  if (target_steps <= 0) {
+    // One render: items of equal size whose count is a whole number (1-8) of rounds over the
+    // CUs - the atomic queue hands equal items out in rounds, and a last round of a few items
+    // leaves the other CUs idle (a plan of 2.03 rounds takes 3).
+    const int cus = device_cus();
+    const double w = static_cast<double>(total_steps) * tiles;
+    const long long rounds = std::max(1LL, std::min(8LL, std::llround(w / (static_cast<double>(floor_steps) * cus))));
+    const long long chunks = std::max(1LL, (rounds * cus + tiles - 1) / tiles);
+    target_steps = (total_steps + chunks - 1) / chunks;
+  }
   int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
   chunk_steps += chunk_steps & 1;  // even: items of the two-node kernel are whole 32-particle stages
   int chunk = 0;

@@ -245,11 +245,10 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 // the next 32-particle stage (bins are padded to 32 particles, so a stage lies in
 // one bin); waves 4-7 multiply the current one: consumer c = wave - 4 holds the
 // 32 x 32 block (c % 2, c / 2) for both nodes - 4 products of accumulators per node
-// and the folded S(+-x) sums.  Producer c and consumer c share a SIMD.  The chunk
-// slots of an item are 2 * chunk + t (node t's x = 0 column; the reduction adds
-// the two) - the consumers write both nodes' sum into slot 2 * chunk and zeros'
-// complement is not needed: slot 2 * chunk + 1 holds only its x = 0 column and
-// zeros elsewhere.
+// and the folded S(+-x) sums.  Producer c and consumer c share a SIMD.  An item owns
+// chunk slots 2 * chunk and 2 * chunk + 1 (the reduction adds them): the consumers
+// write both nodes' folded sums into slot 2 * chunk and zeros into slot 2 * chunk + 1,
+// and the producers write node t's x = 0 column into slot 2 * chunk + t.
 constexpr int kPairStage = 2 * kStep;             // particles per stage
 constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
 constexpr int kPairThreads = 512;
