@@ -2,8 +2,9 @@
 
     python -m libmeshlessvis_amd.build          # or build() from Python
 
-Every source under csrc/ is compiled for gfx950 into _build/ at the repository
-root, then linked into libmeshlessvis_amd/libmeshless_vis.so.  The FP32
+Every library source under csrc/ is compiled for gfx950 into _build/ at the
+repository root, then linked into libmeshlessvis_amd/libmeshless_vis.so; the batch
+renderer csrc/vis_noninteractive.cpp becomes libmeshlessvis_amd/vis_noninteractive.  The FP32
 packed-math feature is switched off for device code (the gfx950 packed FP32
 instructions are kept out of every kernel).
 """
@@ -21,6 +22,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 BUILD = ROOT / "_build" / "libmeshless_vis"
 LIB = PKG / "libmeshless_vis.so"
+CLI = PKG / "vis_noninteractive"
 ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
 
 SOURCES = ["vis_runtime.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
@@ -80,7 +82,22 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"[build] {LIB}")
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """The batch renderer (vis_noninteractive.cpp), linked against the library beside it."""
+    tmp = CLI.with_suffix(".tmp")
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / "vis_noninteractive.cpp"),
+           f"-L{PKG}", "-lmeshless_vis", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed for vis_noninteractive.cpp:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    if verbose:
+        print(f"[build] {CLI}")
+    return CLI
 
 
 if __name__ == "__main__":
