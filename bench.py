@@ -314,6 +314,8 @@ def main() -> None:
                     help="drifting_clusters: skip the per-timestep upload (registration) measurement")
     ap.add_argument("--autotune", type=float, default=0.0, metavar="TOL",
                     help="before timing, let vis_autotune pick nodes/bin ratio within TOL of the plain kernel")
+    ap.add_argument("--cull", action="store_true",
+                    help="render with cull_fully_aliased_terms set (drop terms wholly outside the period window)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -342,7 +344,7 @@ def main() -> None:
 
     ds = host.Dataset([wl.group()])
     cfgs = [host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
-                        phi_mode=wl.phi_mode) for vw in wl.views]
+                        phi_mode=wl.phi_mode, cull=args.cull) for vw in wl.views]
     cfgs[0].register(ds)
     tuned = None
     if args.autotune > 0 and form != "plain":
@@ -440,6 +442,7 @@ def main() -> None:
             "phi_mode": wl.phi_mode,
             "form": form,
             "autotune": tuned,
+            "cull_fully_aliased_terms": bool(args.cull),
         },
         "views_per_s": views / elapsed,
         "rel_l2_vs_fp64": rel,

@@ -81,7 +81,12 @@ typedef struct
 	float3 u_axis;
 	float3 v_axis;
 
-	bool cull_fully_aliased_terms;   /* kept for layout; never read (as in the reference) */
+	bool cull_fully_aliased_terms;   /* The reference never reads it (its culling, meshless_vis.cu:39-127,
+	                                    is dead code).  Here, when true, every render drops the terms
+	                                    whose projected footprint - the square of half-side k * r0
+	                                    (k = 2 SPH, 1 Wendland, 3 Gaussian; r0 = 1 without radii)
+	                                    around (p.u_axis, p.v_axis) - lies wholly outside the image's
+	                                    period window |u| <= 1/(2 step.x), |v| <= 1/(2 step.y).       */
 
 	bool _automatic_d_image;
 

@@ -72,12 +72,19 @@ struct ViewParams {
 
 // ------------------------------------------------------ kernel launchers ---
 // prep.hip
+// cull > 0: a particle whose projected footprint, the square of half-side cull * r0
+// around (p.u, p.v), lies wholly outside the image's period window
+// [-1/(2 sx), 1/(2 sx)] x [-1/(2 sy), 1/(2 sy)] (a fully aliased term) gets weight 0.
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
-                               double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s);
+                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s);
 
 hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
-                                      int n_slots, double sx, double sy, double3 u, double3 v, Packet* out,
-                                      hipStream_t s);
+                                      int n_slots, double sx, double sy, double3 u, double3 v, float cull,
+                                      Packet* out, hipStream_t s);
+
+// The cull factor of a group under a config: 0 unless cull_fully_aliased_terms is set;
+// else the basis's support in units of r0 (SPH 2, Wendland 1, Gaussian 3: exp(-9 pi)).
+float cull_factor(const VisConfig* c, int basis);
 
 // One job of a batched packet preparation: the packets of a (view, group) pair.
 struct PrepJob {
@@ -88,6 +95,8 @@ struct PrepJob {
   int n, n_slots;
   double sx, sy;
   double3 u, v;
+  float cull;  // as launch_prep_packets
+  int pad;
 };
 // Every job's packets in one launch; begin[j] = first flattened slot of job j (prefix sums of
 // n_slots, ascending), total = all slots.

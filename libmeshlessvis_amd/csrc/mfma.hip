@@ -584,10 +584,11 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
     if (pl.bc)
       e = launch_prep_packets_mapped(sp.g->d_constraints, sp.g->d_radii, pl.bc->slot_map.as<int>(), pl.n_slots,
                                      sp.c->step_size.x, sp.c->step_size.y, sp.u, sp.v,
-                                     const_cast<Packet*>(pl.job.packets), s);
+                                     cull_factor(sp.c, sp.g->basis_function_id), const_cast<Packet*>(pl.job.packets), s);
     else
       e = launch_prep_packets(sp.g->d_constraints, sp.g->d_radii, sp.n, pl.n_slots, sp.c->step_size.x,
-                              sp.c->step_size.y, sp.u, sp.v, const_cast<Packet*>(pl.job.packets), s);
+                              sp.c->step_size.y, sp.u, sp.v, cull_factor(sp.c, sp.g->basis_function_id),
+                              const_cast<Packet*>(pl.job.packets), s);
     if (!hip_ok(e, "prep_packets launch")) return false;
   }
   if (!mfma_launch_sums(specs, bt, s, ev)) return false;

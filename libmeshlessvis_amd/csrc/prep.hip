@@ -19,9 +19,12 @@ __device__ __forceinline__ uint32_t to_fixed_rev(double a) {
   return static_cast<uint32_t>(static_cast<uint64_t>(scaled));  // 2^32 wraps to 0
 }
 
-// The packet of particle i (a zero packet for i < 0).
+// The packet of particle i (a zero packet for i < 0).  cull > 0: weight 0 when the
+// footprint square of half-side cull * r0 around (du, dv) misses the period window
+// |du| <= 1/(2 sx), |dv| <= 1/(2 sy) (the image is periodic in u and v with period
+// 1/step, centred on the origin).
 __device__ __forceinline__ Packet make_packet(const Constraint* __restrict__ c, const float* __restrict__ radii, int i,
-                                              double sx, double sy, double3 u, double3 v) {
+                                              double sx, double sy, double3 u, double3 v, float cull) {
   Packet p;
   if (i >= 0) {
     const float4 q = reinterpret_cast<const float4*>(c)[i];
@@ -32,6 +35,10 @@ __device__ __forceinline__ Packet make_packet(const Constraint* __restrict__ c, 
     p.pv = to_fixed_rev(sy * dv);
     p.w = q.w;
     p.h = radii ? radii[i] : 1.0f;
+    if (cull > 0.0f) {
+      const double r = static_cast<double>(cull) * p.h, hu = 0.5 / fabs(sx), hv = 0.5 / fabs(sy);
+      if (du + r < -hu || du - r > hu || dv + r < -hv || dv - r > hv) p.w = 0.0f;
+    }
   } else {
     p.pu = 0u;
     p.pv = 0u;
@@ -43,11 +50,11 @@ __device__ __forceinline__ Packet make_packet(const Constraint* __restrict__ c, 
 
 __global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __restrict__ c,
                                                            const float* __restrict__ radii, int n, int n_padded,
-                                                           double sx, double sy, double3 u, double3 v,
+                                                           double sx, double sy, double3 u, double3 v, float cull,
                                                            Packet* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_padded) return;
-  out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v);
+  out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v, cull);
 }
 
 // Same packets in a given slot order: slot k takes particle slot_map[k] (zero packet when negative).
@@ -55,10 +62,10 @@ __global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constrai
                                                                   const float* __restrict__ radii,
                                                                   const int* __restrict__ slot_map, int n_slots,
                                                                   double sx, double sy, double3 u, double3 v,
-                                                                  Packet* __restrict__ out) {
+                                                                  float cull, Packet* __restrict__ out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_slots) return;
-  out[k] = make_packet(c, radii, slot_map[k], sx, sy, u, v);
+  out[k] = make_packet(c, radii, slot_map[k], sx, sy, u, v, cull);
 }
 
 // All jobs of a batch: thread g of the flattened slot range finds its job by binary search.
@@ -79,7 +86,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restric
   const int k = static_cast<int>(g - begin[lo]);
   if (k >= J.n_slots) return;
   const int i = J.slot_map ? J.slot_map[k] : (k < J.n ? k : -1);
-  J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v);
+  J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v, J.cull);
 }
 
 hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s) {
@@ -90,22 +97,27 @@ hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_j
 }
 
 hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
-                                      int n_slots, double sx, double sy, double3 u, double3 v, Packet* out,
-                                      hipStream_t s) {
+                                      int n_slots, double sx, double sy, double3 u, double3 v, float cull,
+                                      Packet* out, hipStream_t s) {
   if (n_slots <= 0) return hipSuccess;
   hipLaunchKernelGGL(prep_packets_mapped_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, s, d_constraints, d_radii,
-                     slot_map, n_slots, sx, sy, u, v, out);
+                     slot_map, n_slots, sx, sy, u, v, cull, out);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
-                               double sx, double sy, double3 u, double3 v, Packet* out, hipStream_t s) {
+                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s) {
   if (n_padded <= 0) return hipSuccess;
   const int block = 256;
   const int grid = (n_padded + block - 1) / block;
   hipLaunchKernelGGL(prep_packets_kernel, dim3(grid), dim3(block), 0, s, d_constraints, d_radii, n, n_padded, sx,
-                     sy, u, v, out);
+                     sy, u, v, cull, out);
   return hipGetLastError();
+}
+
+float cull_factor(const VisConfig* c, int basis) {
+  if (!c || !c->cull_fully_aliased_terms) return 0.0f;
+  return basis == SPH ? 2.0f : basis == GAUSSIAN ? 3.0f : 1.0f;
 }
 
 }  // namespace mv

@@ -317,7 +317,7 @@ static bool plain_group(VisPlan* p, const MfmaJobSpec& sp, int nfreq) {
   const int n_padded = ((n + 255) / 256) * 256;
   if (!p->packets.reserve(sizeof(Packet) * std::max(n_padded, 256))) return false;
   if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, sp.c->step_size.x, sp.c->step_size.y, sp.u,
-                                  sp.v, p->packets.as<Packet>(), p->stream),
+                                  sp.v, cull_factor(sp.c, g.basis_function_id), p->packets.as<Packet>(), p->stream),
               "prep_packets launch"))
     return false;
   const int ntiles = n_padded / 256;
@@ -398,7 +398,8 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
                     c->v_axis.z, c->h_bin_ratio})
       key_put(&k, f);
     for (int v : {c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y,
-                  c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes})
+                  c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes,
+                  static_cast<int>(c->cull_fully_aliased_terms)})
       key_put(&k, v);
     for (const void* q : {static_cast<const void*>(c->_d_freq_image), static_cast<const void*>(c->_d_image),
                           static_cast<const void*>(c->_d_freq_image_arranged), static_cast<const void*>(c->_plan->tw_x),
@@ -448,6 +449,7 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
     J.sy = sp.c->step_size.y;
     J.u = sp.u;
     J.v = sp.v;
+    J.cull = cull_factor(sp.c, sp.g->basis_function_id);
     bg_begin[k] = acc;
     acc += pl.n_slots;
     jp[k].partial = pl.job.partial;

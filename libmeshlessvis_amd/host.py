@@ -88,6 +88,7 @@ class Config:
         h_bins=0,
         nodes=0,
         h_bin_ratio=0.0,
+        cull=False,
     ):
         L = lib()
         L.vis_clear_last_error()
@@ -110,6 +111,7 @@ class Config:
         self.cs.number_of_h_bins = int(h_bins)
         self.cs.number_of_interpolation_nodes = int(nodes)
         self.cs.h_bin_ratio = float(h_bin_ratio)
+        self.cs.cull_fully_aliased_terms = bool(cull)
 
     # -- fields ------------------------------------------------------------
     @property
