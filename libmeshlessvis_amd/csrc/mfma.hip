@@ -88,9 +88,11 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
 
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
-      const int ptid = wave * 64 + lane;
-      const int pp = ptid & 15;  // particles 2pp, 2pp+1 of the stage
-      const int gg = ptid >> 4;  // 0..15: A rows and B rows gg + 16m (m < 8)
+      const int pp = lane & 15;  // particles 2pp, 2pp+1 of the stage
+      // 0..15: A rows and B rows gg + 16m (m < 8).  A wave's four row groups are 4 apart, so
+      // its stores (rows 20 words apart) hit each of the 32 LDS banks exactly twice; row
+      // groups 1 apart would collide 3-way.
+      const int gg = (lane >> 4) * 4 + wave;
       // Row gg + 16m: phase = P0 + m * 16 * (phase step), exact in 32-bit modular arithmetic.
       const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
       const uint32_t y_first = static_cast<uint32_t>(y0 + gg);

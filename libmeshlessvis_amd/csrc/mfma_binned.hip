@@ -306,9 +306,11 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
 
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
-      const int ptid = wave * 64 + lane;
-      const int pp = ptid & 15;  // particles 2pp, 2pp+1 of the stage
-      const int gg = ptid >> 4;  // 0..15: A rows and B rows gg + 16m (m < 4)
+      const int pp = lane & 15;  // particles 2pp, 2pp+1 of the stage
+      // 0..15: A rows and B rows gg + 16m (m < 4).  A wave's four row groups are 4 apart, so
+      // its stores (rows 20 words apart) hit each of the 32 LDS banks exactly twice; row
+      // groups 1 apart would collide 3-way.
+      const int gg = (lane >> 4) * 4 + wave;
       const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
       const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
       const bool zero_col = tile_x == 0 && sh.cx > 0;  // (cx = 0: an empty half plane)
@@ -367,24 +369,30 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
           store_pair<OP, SPLIT, kA>(reinterpret_cast<uint32_t(*)[kA]>(ldsA(buf, 0)), (gg + 16 * m) * kPairPitch + pp, re,
                                     im);
         }
+        // Node weights w * l_t once per particle, then one multiply per value and node.
+        float wl[2][T];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          wl[u][0] = a.q[u].w * a.l[u].x;
+          wl[u][1] = a.q[u].w * a.l[u].y;
+        }
 #pragma unroll
         for (int m = 0; m < 4; ++m) {  // B_t(i, y)
-          float re0[2], im0[2];
+          float c0[2], s0[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
             const float ph = phase_rev(PB[u]);
             PB[u] += dB[u];
-            re0[u] = a.q[u].w * cos_rev(ph);
-            im0[u] = a.q[u].w * sin_rev(ph);
+            c0[u] = cos_rev(ph);
+            s0[u] = sin_rev(ph);
           }
 #pragma unroll
           for (int t = 0; t < T; ++t) {
             float re[2], im[2];
 #pragma unroll
             for (int u = 0; u < 2; ++u) {
-              const float l = t == 0 ? a.l[u].x : a.l[u].y;
-              re[u] = l * re0[u];
-              im[u] = l * im0[u];
+              re[u] = wl[u][t] * c0[u];
+              im[u] = wl[u][t] * s0[u];
             }
             if (decltype(zero)::value) {
               zr[t][m] += re[0] + re[1];
