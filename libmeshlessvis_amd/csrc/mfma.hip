@@ -440,12 +440,15 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   const int floor_steps = binned ? 128 : 32;
   out->tile_steps = total_steps * tiles;
   if (target_steps <= 0) {
-    // One render: items of equal size whose count is a whole number (1-8) of rounds over the
-    // CUs - the atomic queue hands equal items out in rounds, and a last round of a few items
-    // leaves the other CUs idle (a plan of 2.03 rounds takes 3).
+    // One render: items of equal size whose count is a whole number (1 or 2) of rounds over
+    // the CUs - the atomic queue hands equal items out in rounds, and a last round of a few
+    // items leaves the other CUs idle (a plan of 2.03 rounds takes 3).  Few rounds: every
+    // item writes a partial spectrum tile that the reduction reads back (the 1M-particle
+    // uniform view: 8 rounds 1.72 ms per view, 1.41 of them in the MFMA kernel; 2 rounds
+    // 1.48 ms, 1.38 in the kernel).
     const int cus = device_cus();
     const double w = static_cast<double>(total_steps) * tiles;
-    const long long rounds = std::max(1LL, std::min(8LL, std::llround(w / (static_cast<double>(floor_steps) * cus))));
+    const long long rounds = std::max(1LL, std::min(2LL, std::llround(w / (static_cast<double>(floor_steps) * cus))));
     const long long chunks = std::max(1LL, (rounds * cus + tiles - 1) / tiles);
     target_steps = (total_steps + chunks - 1) / chunks;
   }
