@@ -1,0 +1,73 @@
+"""Spectrum shapes at the edges of the MFMA kernels' tiling, against the FP64 oracle:
+cutoffs that leave partial |x| and y tiles (sum kernel: 128 x 128 tiles; two-node
+binned kernel: 64 x 64; three-node kernel: 128 x-columns x 64), a single |x| column,
+and an empty half plane (cutoff 0).  Block length 64 lets 2cx*cy take these values.
+Tolerance as the other split-FP16 tests: relative L2 <= 1e-5."""
+import numpy as np
+import pytest
+
+import oracle
+from libmeshlessvis_amd import host
+
+pytestmark = pytest.mark.gpu
+
+U = np.array([0.6, 0.8, 0.0])
+V = np.array([-0.48, 0.36, 0.8])
+STEP = (0.5, 0.5)
+
+
+def samples_for(cutoff):
+    m = max(64, 2 * max(cutoff))
+    return (1 << (m - 1).bit_length(),) * 2
+
+
+def render(groups, cutoff, nodes):
+    ds = host.Dataset(groups)
+    with host.Config(cutoff=cutoff, samples=samples_for(cutoff), step=STEP, u=U, v=V, block_length=64,
+                     form="mfma_split_f16", phi_mode="analytic", nodes=nodes) as cfg:
+        assert cfg.check()
+        cfg.register(ds)
+        try:
+            cfg.render(ds)
+            return cfg.spectrum(), cfg.image()
+        finally:
+            cfg.unregister(ds)
+
+
+KINDS = {
+    "gauss_var": ("GAUSSIAN", 0),       # sum kernel (Gaussian with radii)
+    "wendland_t2": ("WENDLAND_D3_C2", 2),  # two-node binned kernel
+    "sph_t3": ("SPH", 3),                # three-node binned kernel
+}
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+@pytest.mark.parametrize("cutoff", [(1, 32), (80, 70), (136, 8), (200, 40)])
+def test_partial_tiles(kind, cutoff):
+    basis, nodes = KINDS[kind]
+    rng = np.random.default_rng(sum(cutoff))
+    n = 2500
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    w = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    h = np.exp(rng.uniform(np.log(0.01), np.log(0.08), n)).astype(np.float32)
+    groups = [(pos, w, h, basis)]
+    print(f"[case] {kind} cutoff={cutoff}", flush=True)
+    F, img = render(groups, cutoff, nodes)
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"rel-L2 {e:.2e}")
+    assert e <= 1e-5
+    img64 = oracle.c2r64(oracle.arrange64(F64, *cutoff, *samples_for(cutoff)), *samples_for(cutoff))
+    assert oracle.rel_l2(img, img64) <= 1e-5
+
+
+@pytest.mark.parametrize("kind", list(KINDS))
+def test_empty_half_plane(kind):
+    basis, nodes = KINDS[kind]
+    rng = np.random.default_rng(3)
+    n = 500
+    groups = [(rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32), rng.uniform(0.1, 1, n).astype(np.float32),
+               rng.uniform(0.01, 0.05, n).astype(np.float32), basis)]
+    F, img = render(groups, (0, 8), nodes)
+    assert F.size == 0
+    assert not img.any()
