@@ -455,7 +455,7 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
   chunk_steps += chunk_steps & 1;  // even: items of the two-node kernel are whole 32-particle stages
   int chunk = 0;
-  auto add_range = [&](int bin, int s0, int s1) {
+  auto add_range = [&](int bin, int s0, int s1, int cost) {
     for (int t = 0; t < tiles; ++t) {
       MfmaItem it{};
       it.tile = t;
@@ -463,19 +463,42 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
       it.s1 = s1;
       it.bin = bin;
       it.chunk = chunk;
+      it.cost = cost;
       out->items.push_back(it);
     }
     ++chunk;
   };
   if (binned) {
-    for (long long s0 = 0; s0 < total_steps; s0 += chunk_steps)
-      add_range(0, static_cast<int>(s0), static_cast<int>(std::min<long long>(total_steps, s0 + chunk_steps)));
+    // Ranges of equal cost rather than equal length: a step costs 1 and every bin start
+    // kFoldCost more (the fold of the accumulators at a bin change costs about three steps),
+    // so the ranges over the sparse tails of the h distribution, one short bin after
+    // another, are not the stragglers of the launch.
+    constexpr double kFoldCost = 3.0;
+    std::vector<double> cum(static_cast<size_t>(total_steps) + 1, 0.0);
+    std::vector<char> starts(static_cast<size_t>(total_steps) + 1, 0);
+    for (size_t b = 0; b + 1 < bin_step.size(); ++b)
+      if (bin_step[b] < bin_step[b + 1]) starts[bin_step[b]] = 1;
+    for (long long k = 0; k < total_steps; ++k) cum[k + 1] = cum[k] + 1.0 + (starts[k] ? kFoldCost : 0.0);
+    const long long n_ranges = std::max<long long>(1, (total_steps + chunk_steps - 1) / chunk_steps);
+    long long s0 = 0;
+    for (long long r = 1; r <= n_ranges && s0 < total_steps; ++r) {
+      long long s1 = total_steps;
+      if (r < n_ranges) {
+        const double goal = cum[total_steps] * static_cast<double>(r) / static_cast<double>(n_ranges);
+        s1 = std::lower_bound(cum.begin(), cum.end(), goal) - cum.begin();
+        s1 += s1 & 1;  // whole 32-particle stages
+        s1 = std::min(total_steps, std::max(s1, s0 + 2));
+      }
+      add_range(0, static_cast<int>(s0), static_cast<int>(s1), static_cast<int>(cum[s1] - cum[s0] + 0.5));
+      s0 = s1;
+    }
   } else {
     for (size_t b = 0; b + 1 < bin_step.size(); ++b)
       for (int s0 = bin_step[b]; s0 < bin_step[b + 1]; s0 += chunk_steps)
-        add_range(static_cast<int>(b), s0, std::min(bin_step[b + 1], s0 + chunk_steps));
+        add_range(static_cast<int>(b), s0, std::min(bin_step[b + 1], s0 + chunk_steps),
+                  std::min(bin_step[b + 1], s0 + chunk_steps) - s0);
   }
-  if (chunk == 0) add_range(0, 0, 0);
+  if (chunk == 0) add_range(0, 0, 0, 0);
   out->nchunks = paired ? 2 * chunk : chunk;  // the two-node kernel writes one slot per node
   return true;
 }
@@ -550,9 +573,8 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
         its.push_back(it);
       }
     }
-    // Longest ranges first: the atomic queue then balances the tail.
-    std::stable_sort(its.begin(), its.end(),
-                     [](const MfmaItem& a, const MfmaItem& b) { return (a.s1 - a.s0) > (b.s1 - b.s0); });
+    // Costliest items first: the atomic queue then balances the tail.
+    std::stable_sort(its.begin(), its.end(), [](const MfmaItem& a, const MfmaItem& b) { return a.cost > b.cost; });
     items.insert(items.end(), its.begin(), its.end());
   }
   bt->job_begin.push_back(static_cast<int>(jobs.size()));

@@ -21,7 +21,8 @@ struct MfmaJob {
 struct MfmaItem {
   int job, tile, s0, s1;  // steps [s0, s1)
   int bin, chunk;         // the range's bin (sum kernel) and its partial slot
-  int pad0, pad1;
+  int cost;               // host-side estimate in steps (queue order); not read by the kernels
+  int pad;
 };
 
 struct MfmaPlanned {
