@@ -463,7 +463,10 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
       it.s1 = s1;
       it.bin = bin;
       it.chunk = chunk;
-      it.cost = cost;
+      // Tiles of the first |x| range also sum the x = 0 column (binned kernels: about a
+      // quarter more generation work per stage): queued first, so that no workgroup gets
+      // two of them in a row while another gets none.
+      it.cost = t % job.tiles_x != 0 ? cost : binned ? cost + cost / 4 : cost + cost / 16;
       out->items.push_back(it);
     }
     ++chunk;
