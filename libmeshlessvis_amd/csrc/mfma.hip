@@ -473,10 +473,12 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   };
   if (binned) {
     // Ranges of equal cost rather than equal length: a step costs 1 and every bin start
-    // kFoldCost more (the fold of the accumulators at a bin change costs about three steps),
-    // so the ranges over the sparse tails of the h distribution, one short bin after
-    // another, are not the stragglers of the launch.
-    constexpr double kFoldCost = 3.0;
+    // kFoldCost more (the fold of the accumulators and of the x = 0 column at a bin change,
+    // and the stage it holds up), so the ranges over the sparse tails of the h distribution,
+    // one short bin after another, are not the stragglers of the launch.  The value is
+    // measured: on the 262k-particle clustered view 2, 3, 5, 8, 12, 20 give 0.378, 0.353,
+    // 0.313, 0.277, 0.276, 0.287 ms.
+    constexpr double kFoldCost = 8.0;
     std::vector<double> cum(static_cast<size_t>(total_steps) + 1, 0.0);
     std::vector<char> starts(static_cast<size_t>(total_steps) + 1, 0);
     for (size_t b = 0; b + 1 < bin_step.size(); ++b)
