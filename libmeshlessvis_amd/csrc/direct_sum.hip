@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) void finalize_spectrum_kernel(const double2* _
 
 // Batched reduction + finalize: block row r = render; partial index f = j * cy + y (j = x + cx - 1)
 // is stored at F[y * 2cx + x0], x0 the reference column of x.
-__global__ __launch_bounds__(256) void batch_finish_kernel(const RenderOut* __restrict__ r,
+__global__ __launch_bounds__(64) void batch_finish_kernel(const RenderOut* __restrict__ r,
                                                            const JobPart* __restrict__ jp, int cx, int cy) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int two_cx = 2 * cx;
@@ -216,11 +216,9 @@ __global__ __launch_bounds__(256) void batch_finish_kernel(const RenderOut* __re
   double re = 0.0, im = 0.0;
   for (int j = ro.job_begin; j < ro.job_end; ++j) {
     const JobPart p = jp[j];
-    for (int k = 0; k < p.nchunks; ++k) {
-      const float2 v = p.partial[static_cast<size_t>(k) * nfreq + f];
-      re += static_cast<double>(v.x);
-      im += static_cast<double>(v.y);
-    }
+    const double2 sum = sum_chunks(p.partial + f, p.nchunks, static_cast<size_t>(nfreq));
+    re += sum.x;
+    im += sum.y;
   }
   const int j = f / cy;
   const int y = f - j * cy;
@@ -233,7 +231,7 @@ __global__ __launch_bounds__(256) void batch_finish_kernel(const RenderOut* __re
 hipError_t launch_batch_finish(const RenderOut* r, const JobPart* jp, int n_renders, int cx, int cy, hipStream_t s) {
   const int nfreq = 2 * cx * cy;
   if (nfreq <= 0 || n_renders <= 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_finish_kernel, dim3((nfreq + 255) / 256, n_renders), dim3(256), 0, s, r, jp, cx, cy);
+  hipLaunchKernelGGL(batch_finish_kernel, dim3((nfreq + 63) / 64, n_renders), dim3(64), 0, s, r, jp, cx, cy);
   return hipGetLastError();
 }
 

@@ -70,6 +70,27 @@ struct ViewParams {
   int phi_mode;        // VisPhiMode
 };
 
+// sum_k p[k * stride] over k < n in FP64, in a fixed order: four interleaved running sums
+// (k mod 4: four loads in flight per thread instead of one), then (s0 + s1) + (s2 + s3).
+__device__ __forceinline__ double2 sum_chunks(const float2* __restrict__ p, int n, size_t stride) {
+  double r[4] = {0.0, 0.0, 0.0, 0.0}, q[4] = {0.0, 0.0, 0.0, 0.0};
+  int k = 0;
+  for (; k + 4 <= n; k += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float2 v = p[static_cast<size_t>(k + u) * stride];
+      r[u] += static_cast<double>(v.x);
+      q[u] += static_cast<double>(v.y);
+    }
+  }
+  for (int u = 0; k < n; ++k, ++u) {
+    const float2 v = p[static_cast<size_t>(k) * stride];
+    r[u] += static_cast<double>(v.x);
+    q[u] += static_cast<double>(v.y);
+  }
+  return make_double2((r[0] + r[1]) + (r[2] + r[3]), (q[0] + q[1]) + (q[2] + q[3]));
+}
+
 // ------------------------------------------------------ kernel launchers ---
 // prep.hip
 // cull > 0: a particle whose projected footprint, the square of half-side cull * r0
@@ -103,7 +124,7 @@ struct PrepJob {
 hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s);
 
 // One render of a batch after the MFMA launches: its jobs' split-K partials
-// (JobPart[job_begin, job_end), summed in order in FP64) give F = scale * (re, -im);
+// (JobPart[job_begin, job_end), summed in FP64 in a fixed order) give F = scale * (re, -im);
 // then the C2R through work into image.
 struct JobPart {
   const float2* partial;  // [nchunks][2cx][cy]

@@ -35,7 +35,7 @@
 //    forms are a sign flip and a 16-bit rotation of the packed (Br, Bi);
 //    split products hi*hi + hi*lo + lo*hi share the FP32 accumulators;
 //  * an item spans at most 16384 particles of FP32 accumulation; its partial
-//    tile (FP32) is reduced with the job's other items in FP64, in order.
+//    tile (FP32) is reduced with the job's other items in FP64, in a fixed order.
 #include "mfma_common.h"
 
 #include <cstring>
@@ -296,19 +296,15 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
   }
 }
 
-// acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (in order, FP64), x0 the reference column of x = j - cx + 1.
-__global__ __launch_bounds__(256) void mfma_reduce_kernel(const float2* __restrict__ partial, int nchunks, int cx,
-                                                          int cy, int first, double2* __restrict__ acc) {
+// acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (FP64, fixed order), x0 the reference column of x = j - cx + 1.
+__global__ __launch_bounds__(64) void mfma_reduce_kernel(const float2* __restrict__ partial, int nchunks, int cx,
+                                                         int cy, int first, double2* __restrict__ acc) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int two_cx = 2 * cx;
   const int nfreq = two_cx * cy;
   if (f >= nfreq) return;
-  double re = 0.0, im = 0.0;
-  for (int k = 0; k < nchunks; ++k) {
-    const float2 v = partial[static_cast<size_t>(k) * nfreq + f];
-    re += static_cast<double>(v.x);
-    im += static_cast<double>(v.y);
-  }
+  const double2 sum = sum_chunks(partial + f, nchunks, static_cast<size_t>(nfreq));
+  double re = sum.x, im = sum.y;
   const int j = f / cy;
   const int y = f - j * cy;
   const int x = j - cx + 1;
@@ -626,7 +622,7 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
   if (!mfma_launch_sums(specs, bt, s, ev)) return false;
   for (size_t k = 0; k < specs.size(); ++k) {
     const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
-    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 255) / 256), dim3(256), 0, s, bt.plans[k].job.partial,
+    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(64), 0, s, bt.plans[k].job.partial,
                        bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
     if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
   }
