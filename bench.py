@@ -32,6 +32,9 @@ PEAKS = {  # MI355X published dense peaks
     "fp16_TFLOPs": 2500.0,
 }
 FLOPS_PER_TERM = 8.0  # one complex multiply-accumulate of w*Phi by exp(-i theta)
+# Arithmetic type of each form's products (the MFMA forms accumulate in FP32; the split forms
+# multiply hi/lo halves of that type; every form reduces its partial sums in FP64).
+DTYPE = {"plain": "f32", "mfma_f16": "f16", "mfma_split_f16": "f16", "mfma_split_bf16": "bf16"}
 
 
 def log(msg: str) -> None:
@@ -251,7 +254,7 @@ def batched_bench(args) -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": form,
+        "dtype": DTYPE[form],
         "data": "synthetic",
         "config": {"workload": "drifting_clusters_16x32", "timesteps": len(wls), "views_per_timestep": 32,
                    "particles_total": n_total, "image": "256x256", "basis": "WENDLAND_D3_C2",
@@ -396,7 +399,7 @@ def main() -> None:
     if form == "plain":
         bound, peak = "valu", PEAKS["fp32_TFLOPs"]
     else:
-        bound, peak = "mfma", PEAKS["bf16_TFLOPs"]
+        bound, peak = "mfma", PEAKS["bf16_TFLOPs" if DTYPE[form] == "bf16" else "fp16_TFLOPs"]
     roofline = {
         "bound": bound,
         "achieved": achieved_tflops,
@@ -430,7 +433,7 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if form == "plain" else form,
+        "dtype": DTYPE[form],
         "data": "synthetic",
         "config": {
             "workload": wl.name,
