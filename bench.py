@@ -173,6 +173,22 @@ def batched_bench(args) -> None:
             meta.append((wl, vw))
     for ds in sets:
         cfgs[0].register(ds)
+    tuned = None
+    if args.autotune > 0 and form != "plain":
+        # One setting for the whole batch (a batch is one graph only when its renders share their
+        # bin settings): tuned on view 0 of the largest timestep.
+        t = time.perf_counter()
+        big = max(range(len(wls)), key=lambda k: wls[k].n)
+        probe = cfgs[big * len(wls[0].views)]
+        tuned_ms = probe.autotune(sets[big], args.autotune)
+        cs = probe.cs
+        tuned = {"tolerance": args.autotune, "nodes": cs.number_of_interpolation_nodes,
+                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "spectrum_ms": tuned_ms, "timestep": big}
+        for c in cfgs:
+            c.cs.number_of_h_bins = 0
+            c.cs.number_of_interpolation_nodes = cs.number_of_interpolation_nodes
+            c.cs.h_bin_ratio = cs.h_bin_ratio
+        log(f"autotune: {tuned} ({time.perf_counter() - t:.1f} s)")
     n_total = sum(wl.n for wl in wls)
     terms = sum(wl.terms_per_view * len(wl.views) for wl in wls)
     log(f"workload drifting_clusters: {len(wls)} timesteps, N={n_total} total, {len(cfgs)} renders/step "
@@ -258,7 +274,7 @@ def batched_bench(args) -> None:
         "data": "synthetic",
         "config": {"workload": "drifting_clusters_16x32", "timesteps": len(wls), "views_per_timestep": 32,
                    "particles_total": n_total, "image": "256x256", "basis": "WENDLAND_D3_C2",
-                   "phi_mode": "analytic", "form": form, "batched": True},
+                   "phi_mode": "analytic", "form": form, "batched": True, "autotune": tuned},
         "views_per_s": len(cfgs) * args.steps / elapsed,
         "rel_l2_vs_fp64": rel,
         "plain_fp32": plain,
