@@ -263,7 +263,6 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
   constexpr int kA = kPairTileX * kPairPitch;  // words of one A part
   constexpr int kB = kPairTileY * kPairPitch;  // words of one B part
   __shared__ uint32_t lds[kPairAWords + kPairBWords];
-  __shared__ float z0[T][kPairTileY][2];  // x = 0 column of the item per node, folded (LDS atomics)
   // The work-item index lives in a padding word (words 16..19 of a row are never read or
   // written by the operand code).
   int* next_item = reinterpret_cast<int*>(&lds[kPairStage / 2]);
@@ -274,7 +273,6 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform (scalar register)
   const int two_cx = 2 * sh.cx;
-  for (int i = tid; i < T * kPairTileY * 2; i += kPairThreads) (&z0[0][0][0])[i] = 0.0f;  // ordered by the loop's barrier
 
   for (;;) {
     if (tid == 0) *next_item = atomicAdd(counter, 1);
@@ -326,12 +324,15 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
           a->l[u] = load_float2(job.lag + slot + u);
         }
       };
-      // x = 0: this lane's column sums of B_t per column m (current bin); folded into z0.
-      float zr[T][4], zi[T][4];
+      // x = 0: this lane's column sums of B_t per column m over the current bin (zr, zi), and
+      // their node-weighted sum over the item's bins (Zr, Zi).  Each column's 16 lanes
+      // (pp = 0..15) are summed once, at the end of the item, by shuffles in a fixed order:
+      // every column has one owner row group, so there is nothing to share through LDS.
+      float zr[T][4], zi[T][4], Zr[T][4], Zi[T][4];
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
-        for (int t = 0; t < T; ++t) zr[t][m] = zi[t][m] = 0.0f;
+        for (int t = 0; t < T; ++t) zr[t][m] = zi[t][m] = Zr[t][m] = Zi[t][m] = 0.0f;
       auto zfold = [&](int b) {
         const float* nd = job.nodes + b * kMaxNodes;
         const float ws = job.bininfo[b].z;
@@ -341,8 +342,8 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
 #pragma unroll
           for (int t = 0; t < T; ++t) {
             const float f = phi<BASIS, MODE>(fabsf(fv) * nd[t]) * ws;
-            atomicAdd(&z0[t][gg + 16 * m][0], f * zr[t][m]);
-            atomicAdd(&z0[t][gg + 16 * m][1], f * zi[t][m]);
+            Zr[t][m] = fmaf(f, zr[t][m], Zr[t][m]);
+            Zi[t][m] = fmaf(f, zi[t][m], Zi[t][m]);
             zr[t][m] = zi[t][m] = 0.0f;
           }
         }
@@ -441,19 +442,23 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
         produce(std::true_type{});
       else
         produce(std::false_type{});
-      __syncthreads();  // every producer lane's fold is in z0
-      if (zero_col && pp == 0) {
-        // One lane per column writes node t's x = 0 value into chunk slot 2 * chunk + t,
-        // then clears the column for the next item.
+      __syncthreads();  // as the consumers' end-of-item barrier
+      if (zero_col) {
+        // Sum over the 16 lanes of a column (pp = 0..15), then one lane per column writes node
+        // t's x = 0 value into chunk slot 2 * chunk + t.
 #pragma unroll
         for (int t = 0; t < T; ++t)
 #pragma unroll
           for (int m = 0; m < 4; ++m) {
-            const int col = gg + 16 * m, y = y0 + col;
-            if (y < sh.cy)
+#pragma unroll
+            for (int d = 1; d < 16; d <<= 1) {
+              Zr[t][m] += __shfl_xor(Zr[t][m], d);
+              Zi[t][m] += __shfl_xor(Zi[t][m], d);
+            }
+            const int y = y0 + gg + 16 * m;
+            if (pp == 0 && y < sh.cy)
               job.partial[(static_cast<size_t>(2 * item.chunk + t) * two_cx + (sh.cx - 1)) * sh.cy + y] =
-                  make_float2(z0[t][col][0], z0[t][col][1]);
-            z0[t][col][0] = z0[t][col][1] = 0.0f;
+                  make_float2(Zr[t][m], Zi[t][m]);
           }
       }
       continue;
@@ -595,7 +600,7 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
         }
       }
     }
-    __syncthreads();  // with the producers' x = 0 read-out
+    __syncthreads();  // end of the item: the producers wait here too
   }
 }
 
