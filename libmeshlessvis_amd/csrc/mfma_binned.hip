@@ -241,17 +241,17 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 // the MFMA's K index is the particle.  x = 0 (A = 1) is the column sum of B_t,
 // kept by the waves that generate B.
 //
-// Tile: 64 |x| rows x 64 columns, 8 waves.  Waves 0-3 generate the operands of
+// Tile: 64 |x| rows x 64 columns, 12 waves.  Waves 0-3 generate the operands of
 // the next 32-particle stage (bins are padded to 32 particles, so a stage lies in
-// one bin); waves 4-7 multiply the current one: consumer c = wave - 4 holds the
-// 32 x 32 block (c % 2, c / 2) for both nodes - 4 products of accumulators per node
-// and the folded S(+-x) sums.  Producer c and consumer c share a SIMD.  An item owns
-// chunk slots 2 * chunk and 2 * chunk + 1 (the reduction adds them): the consumers
-// write both nodes' folded sums into slot 2 * chunk and zeros into slot 2 * chunk + 1,
-// and the producers write node t's x = 0 column into slot 2 * chunk + t.
+// one bin); waves 4-11 multiply the current one: consumer c = wave - 4 holds node
+// t = c / 4 of the 32 x 32 block c % 4 - 4 products of accumulators and the folded
+// S(+-x) sums.  Waves w, w + 4 and w + 8 share a SIMD: one producer and both nodes'
+// consumers of block w.  An item owns chunk slots 2 * chunk and 2 * chunk + 1 (the
+// reduction adds them): node t's consumers write its folded sums into slot
+// 2 * chunk + t and its producers the x = 0 column.
 constexpr int kPairStage = 2 * kStep;             // particles per stage
 constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
-constexpr int kPairThreads = 512;
+constexpr int kPairThreads = 768;  // 4 producer waves, 8 consumer waves (block x node)
 constexpr int kPairAWords = 2 * 4 * kPairTileX * kPairPitch;      // [buf][re_hi, re_lo, im_hi, im_lo][row]
 constexpr int kPairBWords = 2 * 2 * 4 * kPairTileY * kPairPitch;  // [buf][node][...][row]
 
@@ -464,38 +464,37 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
       continue;
     }
 
-    // ---- consumer: both nodes of a 32 |x| x 32 y block ----
-    const int c = wave - kProducerWaves;  // 0..3
-    const int wx = c & 1;
-    const int wy = c >> 1;
-    f32x16_t acc[T][4], sp[2], sm[2];  // [node][Prr, Pii, Pri, Pir]; folded S(+x), S(-x)
-#pragma unroll
-    for (int t = 0; t < T; ++t) acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = (f32x16_t){};
+    // ---- consumer: node t of a 32 |x| x 32 y block ----
+    // Waves w, w + 4 and w + 8 share SIMD w: the producer w and the two consumers of block w
+    // (nodes 0 and 1), so every SIMD interleaves one wave's operand generation with two
+    // waves' MFMAs and folds.
+    const int c = wave - kProducerWaves;  // 0..7
+    const int blk = c & 3, t = c >> 2;
+    const int wx = blk & 1;
+    const int wy = blk >> 1;
+    f32x16_t acc[4], sp[2], sm[2];  // [Prr, Pii, Pri, Pir] of node t; folded S(+x), S(-x)
+    acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
     sp[0] = sp[1] = sm[0] = sm[1] = (f32x16_t){};
 
     const int y = y0 + wy * 32 + (lane & 31);
     const float fv = sh.sy * static_cast<float>(y);
     const int xl = xa0 + wx * 32 + (lane >> 5) * 4;  // |x| of output i is xl + (i >> 2) * 8 + (i & 3)
-    // S(+-x) += W_b sum_t Phi(r h_bt) S_t(+-x) of the accumulators; acc = 0.
+    // S(+-x) += W_b Phi(r h_bt) S_t(+-x) of the accumulators; acc = 0.
     auto fold = [&](int b) {
-      const float* nd = job.nodes + b * kMaxNodes;
+      const float hn = job.nodes[b * kMaxNodes + t];
       const float ws = job.bininfo[b].z;
 #pragma unroll
-      for (int t = 0; t < T; ++t) {
-        const float hn = nd[t];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-          const float fu = sh.sx * static_cast<float>(xl + (i >> 2) * 8 + (i & 3));
-          const float f = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * hn) * ws;
-          const float prr = acc[t][0][i], pii = acc[t][1][i], pri = acc[t][2][i], pir = acc[t][3][i];
-          sp[0][i] = fmaf(f, prr - pii, sp[0][i]);
-          sp[1][i] = fmaf(f, pri + pir, sp[1][i]);
-          sm[0][i] = fmaf(f, prr + pii, sm[0][i]);
-          sm[1][i] = fmaf(f, pri - pir, sm[1][i]);
-          __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
-        }
-        acc[t][0] = acc[t][1] = acc[t][2] = acc[t][3] = (f32x16_t){};
+      for (int i = 0; i < 16; ++i) {
+        const float fu = sh.sx * static_cast<float>(xl + (i >> 2) * 8 + (i & 3));
+        const float f = phi<BASIS, MODE>(sqrtf(fu * fu + fv * fv) * hn) * ws;
+        const float prr = acc[0][i], pii = acc[1][i], pri = acc[2][i], pir = acc[3][i];
+        sp[0][i] = fmaf(f, prr - pii, sp[0][i]);
+        sp[1][i] = fmaf(f, pri + pir, sp[1][i]);
+        sm[0][i] = fmaf(f, prr + pii, sm[0][i]);
+        sm[1][i] = fmaf(f, pri - pir, sm[1][i]);
+        __builtin_amdgcn_sched_barrier(0);  // one output at a time: keeps the fold's registers few
       }
+      acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
     };
 
     __syncthreads();
@@ -509,95 +508,49 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
       }
       const int buf = k & 1;
       const uint32_t* A = ldsA(buf, 0);
+      const uint32_t* B = ldsB(buf, t, 0);
       // The lane's LDS offsets are recomputed every stage from the lane id (a few VALU
-      // instructions) rather than kept live across the fold, where they would spill.
-      int lid = static_cast<int>(__lane_id());
-      asm volatile("" : "+v"(lid));
+      // instructions) rather than kept live across the fold, where they would spill; the
+      // lane id comes from a volatile v_mbcnt pair, which the compiler cannot hoist (a hoisted
+      // lane id is spilled, and its reload stalls every stage).
+      int lid;
+      asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(lid));
       const int rowa = (wx * 32 + (lid & 31)) * kPairPitch + (lid >> 5) * 4;
       const int rowb = (wy * 32 + (lid & 31)) * kPairPitch + (lid >> 5) * 4;
       auto lda = [&](int kk, int part) { return *reinterpret_cast<const u32x4_t*>(A + part * kA + rowa + kk * 8); };
-      auto ldb = [&](int kk, int t, int part) {
-        return *reinterpret_cast<const u32x4_t*>(ldsB(buf, t, part) + rowb + kk * 8);
+      auto ldb = [&](int kk, int part) { return *reinterpret_cast<const u32x4_t*>(B + part * kB + rowb + kk * 8); };
+      // acc += (x_re, x_im) x (y_re, y_im): the four real products.
+      auto quart = [&](const u32x4_t& xr, const u32x4_t& xi, const u32x4_t& yr, const u32x4_t& yi) {
+        acc[0] = mfma<OP>(xr, yr, acc[0]);
+        acc[1] = mfma<OP>(xi, yi, acc[1]);
+        acc[2] = mfma<OP>(xr, yi, acc[2]);
+        acc[3] = mfma<OP>(xi, yr, acc[3]);
       };
-      // acc[t] += (x_re, x_im) x (y_re, y_im): the four real products.
-      auto quart = [&](f32x16_t* a, const u32x4_t& xr, const u32x4_t& xi, const u32x4_t& yr, const u32x4_t& yi) {
-        a[0] = mfma<OP>(xr, yr, a[0]);
-        a[1] = mfma<OP>(xi, yi, a[1]);
-        a[2] = mfma<OP>(xr, yi, a[2]);
-        a[3] = mfma<OP>(xi, yr, a[3]);
-      };
-      if (SPLIT) {
-        // Twelve quartets over (kk, node) groups g0 = (0,0), g1 = (0,1), g2 = (1,0), g3 = (1,1),
-        // each split product hi.hi, lo.hi, hi.lo one quartet.  Operands live in four
-        // register pairs - A hi (ah), A lo (al), B hi (bh), B lo (bl) - and every load after
-        // the first four goes into a pair the previous quartet freed, one quartet before
-        // its use, so the LDS latency hides behind MFMAs (the sched_barriers pin the order).
-        u32x4_t ahr = lda(0, 0), ahi = lda(0, 2), bhr = ldb(0, 0, 0), bhi = ldb(0, 0, 2);
-        u32x4_t alr = lda(0, 1), ali = lda(0, 3), blr = ldb(0, 0, 1), bli = ldb(0, 0, 3);
-        auto pin = [] { __builtin_amdgcn_sched_barrier(0); };
-        quart(acc[0], ahr, ahi, bhr, bhi);  // g0 hi.hi
-        pin();
-        quart(acc[0], alr, ali, bhr, bhi);  // g0 lo.hi
-        pin();
-        bhr = ldb(0, 1, 0), bhi = ldb(0, 1, 2);
-        pin();
-        quart(acc[0], ahr, ahi, blr, bli);  // g0 hi.lo
-        pin();
-        blr = ldb(0, 1, 1), bli = ldb(0, 1, 3);
-        pin();
-        quart(acc[1], ahr, ahi, bhr, bhi);  // g1 hi.hi
-        pin();
-        quart(acc[1], ahr, ahi, blr, bli);  // g1 hi.lo
-        pin();
-        ahr = lda(1, 0), ahi = lda(1, 2), blr = ldb(1, 0, 1), bli = ldb(1, 0, 3);
-        pin();
-        quart(acc[1], alr, ali, bhr, bhi);  // g1 lo.hi
-        pin();
-        alr = lda(1, 1), ali = lda(1, 3), bhr = ldb(1, 0, 0), bhi = ldb(1, 0, 2);
-        pin();
-        quart(acc[0], ahr, ahi, blr, bli);  // g2 hi.lo
-        pin();
-        blr = ldb(1, 1, 1), bli = ldb(1, 1, 3);
-        pin();
-        quart(acc[0], ahr, ahi, bhr, bhi);  // g2 hi.hi
-        pin();
-        quart(acc[0], alr, ali, bhr, bhi);  // g2 lo.hi
-        pin();
-        bhr = ldb(1, 1, 0), bhi = ldb(1, 1, 2);
-        pin();
-        quart(acc[1], ahr, ahi, blr, bli);  // g3 hi.lo
-        pin();
-        quart(acc[1], ahr, ahi, bhr, bhi);  // g3 hi.hi
-        pin();
-        quart(acc[1], alr, ali, bhr, bhi);  // g3 lo.hi
-        pin();
-      } else {
 #pragma unroll
-        for (int kk = 0; kk < kPairStage / 16; ++kk) {
-          const u32x4_t ar = lda(kk, 0), ai = lda(kk, 2);
-#pragma unroll
-          for (int t = 0; t < T; ++t) quart(acc[t], ar, ai, ldb(kk, t, 0), ldb(kk, t, 2));
+      for (int kk = 0; kk < kPairStage / 16; ++kk) {
+        const u32x4_t ahr = lda(kk, 0), ahi = lda(kk, 2), bhr = ldb(kk, 0), bhi = ldb(kk, 2);
+        quart(ahr, ahi, bhr, bhi);  // hi.hi
+        if (SPLIT) {
+          const u32x4_t alr = lda(kk, 1), ali = lda(kk, 3);
+          quart(alr, ali, bhr, bhi);  // lo.hi
+          const u32x4_t blr = ldb(kk, 1), bli = ldb(kk, 3);
+          quart(ahr, ahi, blr, bli);  // hi.lo
         }
       }
       since += 2;
       __syncthreads();
     }
     if (n_stages > 0) fold(cur);
-    // The sums into chunk slot 2 * chunk, zeros into slot 2 * chunk + 1 (every item writes its
-    // whole tiles, also without particles).
+    // Node t's sums into chunk slot 2 * chunk + t (every item writes its whole tiles, also
+    // without particles).
     if (y < sh.cy) {
-      float2* out0 = job.partial + static_cast<size_t>(2 * item.chunk) * two_cx * sh.cy + y;
-      float2* out1 = out0 + static_cast<size_t>(two_cx) * sh.cy;
+      float2* out = job.partial + static_cast<size_t>(2 * item.chunk + t) * two_cx * sh.cy + y;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
         const int xa = xl + (i >> 2) * 8 + (i & 3);
         if (xa > sh.cx) continue;
-        out0[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(sp[0][i], sp[1][i]);
-        out1[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(0.0f, 0.0f);
-        if (xa < sh.cx) {
-          out0[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(sm[0][i], sm[1][i]);
-          out1[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(0.0f, 0.0f);
-        }
+        out[static_cast<size_t>(xa + sh.cx - 1) * sh.cy] = make_float2(sp[0][i], sp[1][i]);
+        if (xa < sh.cx) out[static_cast<size_t>(sh.cx - 1 - xa) * sh.cy] = make_float2(sm[0][i], sm[1][i]);
       }
     }
     __syncthreads();  // end of the item: the producers wait here too
