@@ -157,6 +157,9 @@ hipError_t launch_finalize_spectrum(const double2* acc, int nfreq, float scale, 
 
 // fft.hip: 2-D C2R of the half-plane spectrum F[cy][2cx] into image[Nx][Ny];
 // work is the Nx*(Ny/2+1) complex buffer (_d_freq_image_arranged).
+// Zero `words` 32-bit words at p with a kernel (no memset node when a caller captures the stream).
+hipError_t launch_zero_words(void* p, size_t words, hipStream_t s);
+
 hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const float2* tw_x, const float2* tw_y,
                       float2* work, float* image, hipStream_t s);
 

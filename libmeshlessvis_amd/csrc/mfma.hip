@@ -317,9 +317,20 @@ __global__ __launch_bounds__(64) void mfma_reduce_kernel(const float2* __restric
   acc[o] = make_double2(re, im);
 }
 
-__global__ void zero_ints_kernel(int* p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
+// Zero fill by a kernel (grid-stride), so that code a caller captures into a HIP graph records
+// no memset node.
+__global__ void zero_ints_kernel(int* p, size_t n) {
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    p[i] = 0;
+}
+
+hipError_t launch_zero_words(void* p, size_t words, hipStream_t s) {
+  if (words == 0) return hipSuccess;
+  const size_t blocks = std::min<size_t>(1024, (words + 255) / 256);
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, static_cast<int*>(p),
+                     words);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ host ---
@@ -632,8 +643,7 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
 bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
   if (specs.empty()) return true;
   const VisConfig* c0 = specs[0].c;
-  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, s, bt.counters, static_cast<int>(bt.keys.size()));
-  if (!hip_ok(hipGetLastError(), "zero counters")) return false;
+  if (!hip_ok(launch_zero_words(bt.counters, bt.keys.size(), s), "zero counters")) return false;
 
   MfmaShared sh{};
   sh.cx = specs[0].vp.cx;

@@ -828,7 +828,7 @@ void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
   const size_t nfreq = spectrum_count(c);
   if (ds->number_of_groups < 1) {
     // No groups: the reference leaves the spectrum as it was; an empty dataset renders zero here.
-    if (nfreq && !hip_ok(hipMemsetAsync(c->_d_freq_image, 0, sizeof(float2) * nfreq, p->stream), "hipMemset")) return;
+    if (!hip_ok(launch_zero_words(c->_d_freq_image, 2 * nfreq, p->stream), "zero spectrum")) return;
   } else {
     vis_config_compute_scale(c);
     if (!spectrum(ds, c)) return;

@@ -108,6 +108,25 @@ def test_multigroup_partial_sums_and_empty_group():
     assert oracle.rel_l2(img, img64) <= 1e-5
 
 
+@pytest.mark.parametrize("form", ["plain", "mfma_split_f16"])
+def test_dataset_without_groups_renders_zero(form):
+    """A dataset with no groups renders a zero spectrum and image, also right after a non-empty
+    render on the same config (the spectrum buffer is cleared on the config's stream)."""
+    rng = np.random.default_rng(8)
+    full = host.Dataset(_small_groups(rng, 200, "GAUSSIAN", "var"))
+    empty = host.Dataset([])
+    with host.Config(cutoff=(16, 16), samples=(64, 64), form=form) as cfg:
+        cfg.register(full)
+        cfg.render(full)
+        assert np.abs(cfg.spectrum()).max() > 0
+        cfg.unregister(full)
+        cfg.register(empty)
+        cfg.render(empty)
+        F, img = cfg.spectrum(), cfg.image()
+        cfg.unregister(empty)
+    assert not np.any(F) and not np.any(img)
+
+
 @pytest.mark.parametrize("samples", [(96, 80), (64, 256), (1024, 1024)])
 def test_fft_sizes(samples):
     rng = np.random.default_rng(3)
