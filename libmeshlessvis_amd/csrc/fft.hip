@@ -175,12 +175,16 @@ static int ilog2_pow2(int n) {
 hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const float2* tw_x, const float2* tw_y,
                       float2* work, float* image, hipStream_t s) {
   if (Nx > kFftLdsComplex || Ny > kFftLdsComplex) return hipErrorInvalidValue;
+  // A single render's passes have only cy and Nx lines: as many lines per workgroup as the LDS
+  // holds would leave most CUs idle (256x256: 8 and 16 workgroups), so a workgroup takes just
+  // enough lines for about kC2rBlocks workgroups per pass.
+  constexpr int kC2rBlocks = 512;
   if (cy > 0) {
-    const int lines = max(1, min(cy, kFftLdsComplex / Nx));
+    const int lines = max(1, min(min(cy, kFftLdsComplex / Nx), (cy + kC2rBlocks - 1) / kC2rBlocks));
     hipLaunchKernelGGL(c2r_cols_kernel, dim3((cy + lines - 1) / lines), dim3(kFftThreads), 0, s, F, cx, cy, Nx,
                        ilog2_pow2(Nx), Ny, lines, tw_x, work);
   }
-  const int rows = max(1, min(Nx, kFftLdsComplex / Ny));
+  const int rows = max(1, min(min(Nx, kFftLdsComplex / Ny), (Nx + kC2rBlocks - 1) / kC2rBlocks));
   hipLaunchKernelGGL(c2r_rows_kernel, dim3((Nx + rows - 1) / rows), dim3(kFftThreads), 0, s, work, cy, Nx, Ny,
                      ilog2_pow2(Ny), rows, tw_y, image);
   return hipGetLastError();
