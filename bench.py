@@ -327,6 +327,8 @@ def main() -> None:
     ap.add_argument("--form", default="auto")
     ap.add_argument("--views", type=int, default=0, help="use only the first K views (0: all)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP32 comparison run")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched-call timing of the same views (one graph per step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-animation", action="store_true",
@@ -384,6 +386,23 @@ def main() -> None:
     value = terms / elapsed
     log(f"{form}: {elapsed:.3f} s for {views} views -> {value:.3e} terms/s; kernel {kernel_ms / max(1, launches):.3f} ms/launch")
 
+    batched = None
+    if form != "plain" and len(cfgs) > 1 and not args.no_batched:
+        # The same views as one batched call: one captured graph, one persistent launch over all views.
+        for _ in range(max(1, args.warmup)):
+            host.render_batched([ds] * len(cfgs), cfgs)
+        for c in cfgs:
+            c.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            host.render_batched([ds] * len(cfgs), cfgs)
+        for c in cfgs:
+            c.synchronize()
+        b_elapsed = time.perf_counter() - t0
+        batched = {"value": terms / b_elapsed, "unit": "terms/s", "ms_per_step": b_elapsed / args.steps * 1e3,
+                   "views_per_step": len(cfgs)}
+        log(f"{form} batched ({len(cfgs)} views per call): {terms / b_elapsed:.3e} terms/s")
+
     plain = None
     if form != "plain" and not args.no_plain:
         pcfg = host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=wl.views[0].u, v=wl.views[0].v,
@@ -393,6 +412,8 @@ def main() -> None:
         p_value = wl.terms_per_view / p_elapsed
         plain = {"value": p_value, "unit": "terms/s", "kernel_ms_per_launch": p_ms / max(1, p_launches),
                  "views": 1, "speedup": value / p_value}
+        if batched:
+            batched["speedup_vs_plain"] = batched["value"] / p_value
         log(f"plain FP32: {p_value:.3e} terms/s ({p_ms / max(1, p_launches):.1f} ms/view); "
             f"{form} is {value / p_value:.1f}x")
 
@@ -465,6 +486,7 @@ def main() -> None:
         },
         "views_per_s": views / elapsed,
         "rel_l2_vs_fp64": rel,
+        "batched": batched,
         "plain_fp32": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
