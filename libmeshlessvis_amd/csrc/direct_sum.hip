@@ -204,22 +204,25 @@ __global__ __launch_bounds__(256) void finalize_spectrum_kernel(const double2* _
   F[i] = make_float2(static_cast<float>(a.x * sc), static_cast<float>(-a.y * sc));
 }
 
-// Batched reduction + finalize: block row r = render; partial index f = j * cy + y (j = x + cx - 1)
-// is stored at F[y * 2cx + x0], x0 the reference column of x.
-__global__ __launch_bounds__(64) void batch_finish_kernel(const RenderOut* __restrict__ r,
-                                                           const JobPart* __restrict__ jp, int cx, int cy) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+// Batched reduction + finalize: block row r = render, 64 frequencies per block (four waves share
+// each frequency's chunks, sum_chunks_block); partial index f = j * cy + y (j = x + cx - 1) is
+// stored at F[y * 2cx + x0], x0 the reference column of x.
+__global__ __launch_bounds__(256) void batch_finish_kernel(const RenderOut* __restrict__ r,
+                                                            const JobPart* __restrict__ jp, int cx, int cy) {
+  __shared__ double2 lds[256];
+  const int f = blockIdx.x * 64 + (threadIdx.x & 63);
   const int two_cx = 2 * cx;
   const int nfreq = two_cx * cy;
-  if (f >= nfreq) return;
+  const bool valid = f < nfreq;
   const RenderOut ro = r[blockIdx.y];
   double re = 0.0, im = 0.0;
-  for (int j = ro.job_begin; j < ro.job_end; ++j) {
+  for (int j = ro.job_begin; j < ro.job_end; ++j) {  // block-uniform: every thread reaches the barriers
     const JobPart p = jp[j];
-    const double2 sum = sum_chunks(p.partial + f, p.nchunks, static_cast<size_t>(nfreq));
+    const double2 sum = sum_chunks_block(p.partial + (valid ? f : 0), p.nchunks, static_cast<size_t>(nfreq), valid, lds);
     re += sum.x;
     im += sum.y;
   }
+  if (threadIdx.x >= 64 || !valid) return;
   const int j = f / cy;
   const int y = f - j * cy;
   const int x = j - cx + 1;
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(64) void batch_finish_kernel(const RenderOut* __res
 hipError_t launch_batch_finish(const RenderOut* r, const JobPart* jp, int n_renders, int cx, int cy, hipStream_t s) {
   const int nfreq = 2 * cx * cy;
   if (nfreq <= 0 || n_renders <= 0) return hipSuccess;
-  hipLaunchKernelGGL(batch_finish_kernel, dim3((nfreq + 63) / 64, n_renders), dim3(64), 0, s, r, jp, cx, cy);
+  hipLaunchKernelGGL(batch_finish_kernel, dim3((nfreq + 63) / 64, n_renders), dim3(256), 0, s, r, jp, cx, cy);
   return hipGetLastError();
 }
 

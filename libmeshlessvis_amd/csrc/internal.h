@@ -70,25 +70,33 @@ struct ViewParams {
   int phi_mode;        // VisPhiMode
 };
 
-// sum_k p[k * stride] over k < n in FP64, in a fixed order: four interleaved running sums
-// (k mod 4: four loads in flight per thread instead of one), then (s0 + s1) + (s2 + s3).
-__device__ __forceinline__ double2 sum_chunks(const float2* __restrict__ p, int n, size_t stride) {
-  double r[4] = {0.0, 0.0, 0.0, 0.0}, q[4] = {0.0, 0.0, 0.0, 0.0};
-  int k = 0;
-  for (; k + 4 <= n; k += 4) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float2 v = p[static_cast<size_t>(k + u) * stride];
-      r[u] += static_cast<double>(v.x);
-      q[u] += static_cast<double>(v.y);
+// sum_k p[k * stride] over k < n in FP64, in a fixed order (deterministic run to run): four
+// running sums s_q over the chunks k = q mod 4, each in increasing k, then (s0 + s1) + (s2 + s3).
+// The four waves of a 256-thread block share the work, one frequency per lane: wave q keeps
+// s_q and the block adds the four through LDS, so four waves' loads are in flight per frequency.
+// Every thread of the block calls it (it holds two barriers); `valid` lanes read p, the others
+// add nothing.  The sum is returned in wave 0 (threads 0..63); lds holds 256 double2.
+__device__ __forceinline__ double2 sum_chunks_block(const float2* __restrict__ p, int n, size_t stride, bool valid,
+                                                    double2* lds) {
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double r = 0.0, s = 0.0;
+  if (valid) {
+#pragma unroll 4
+    for (int k = q; k < n; k += 4) {
+      const float2 v = p[static_cast<size_t>(k) * stride];
+      r += static_cast<double>(v.x);
+      s += static_cast<double>(v.y);
     }
   }
-  for (int u = 0; k < n; ++k, ++u) {
-    const float2 v = p[static_cast<size_t>(k) * stride];
-    r[u] += static_cast<double>(v.x);
-    q[u] += static_cast<double>(v.y);
+  lds[q * 64 + lane] = make_double2(r, s);
+  __syncthreads();
+  double2 out = make_double2(0.0, 0.0);
+  if (q == 0) {
+    const double2 a = lds[lane], b = lds[64 + lane], c = lds[128 + lane], d = lds[192 + lane];
+    out = make_double2((a.x + b.x) + (c.x + d.x), (a.y + b.y) + (c.y + d.y));
   }
-  return make_double2((r[0] + r[1]) + (r[2] + r[3]), (q[0] + q[1]) + (q[2] + q[3]));
+  __syncthreads();  // lds free for the next call
+  return out;
 }
 
 // ------------------------------------------------------ kernel launchers ---

@@ -297,13 +297,16 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
 }
 
 // acc[y*2cx + x0] (+)= sum_k partial[k][j][y] (FP64, fixed order), x0 the reference column of x = j - cx + 1.
-__global__ __launch_bounds__(64) void mfma_reduce_kernel(const float2* __restrict__ partial, int nchunks, int cx,
-                                                         int cy, int first, double2* __restrict__ acc) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+// 64 frequencies per block, four waves sharing each frequency's chunks (sum_chunks_block).
+__global__ __launch_bounds__(256) void mfma_reduce_kernel(const float2* __restrict__ partial, int nchunks, int cx,
+                                                          int cy, int first, double2* __restrict__ acc) {
+  __shared__ double2 lds[256];
+  const int f = blockIdx.x * 64 + (threadIdx.x & 63);
   const int two_cx = 2 * cx;
   const int nfreq = two_cx * cy;
-  if (f >= nfreq) return;
-  const double2 sum = sum_chunks(partial + f, nchunks, static_cast<size_t>(nfreq));
+  const bool valid = f < nfreq;
+  const double2 sum = sum_chunks_block(partial + (valid ? f : 0), nchunks, static_cast<size_t>(nfreq), valid, lds);
+  if (threadIdx.x >= 64 || !valid) return;
   double re = sum.x, im = sum.y;
   const int j = f / cy;
   const int y = f - j * cy;
@@ -633,7 +636,7 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
   if (!mfma_launch_sums(specs, bt, s, ev)) return false;
   for (size_t k = 0; k < specs.size(); ++k) {
     const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
-    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(64), 0, s, bt.plans[k].job.partial,
+    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(256), 0, s, bt.plans[k].job.partial,
                        bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
     if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
   }
