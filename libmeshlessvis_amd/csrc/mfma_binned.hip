@@ -19,9 +19,10 @@
 // bounded by max|Phi^(T)| (r dh)^T / (T! 2^(2T-1)); with bins of ratio 1.1 and
 // T = 4 it is below 2e-6 of Phi(0).
 //
-// Kernel: 8 waves, 128 (x) x 64 (y) complex tile per workgroup, 32 x 32 per
-// wave, T accumulator pairs per wave; operand generation and split products
-// as in mfma.hip.
+// mfma_binned_kernel (T >= 3): 8 waves, 128 (x) x 64 (y) complex tile per
+// workgroup, 32 x 32 per wave, T accumulator pairs per wave; operand generation
+// and split products as in mfma.hip.  T = 2 (the default) runs
+// mfma_binned_pair_kernel below, with its own layout.
 #include "mfma_common.h"
 
 #include <algorithm>
