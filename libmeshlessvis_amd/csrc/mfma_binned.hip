@@ -605,12 +605,15 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   std::vector<double> edge(nb + 1);
   for (int b = 0; b <= nb; ++b)
     edge[b] = logb ? lo * std::pow(hi / lo, static_cast<double>(b) / nb) : lo + (hi - lo) * b / nb;
+  const double log_ratio = logb ? std::log(hi / lo) : 1.0;
   auto bin_of = [&](float x) {
     if (!(hi > lo)) return 0;
-    const double f = logb ? std::log(x / lo) / std::log(hi / lo) : (x - lo) / (hi - lo);
+    const double f = logb ? std::log(x / lo) / log_ratio : (x - lo) / (hi - lo);
     const int b = static_cast<int>(f * nb);
     return std::min(nb - 1, std::max(0, b));
   };
+  // Each particle's bin, once (one logarithm per particle).
+  std::vector<uint16_t> bin(n);
   std::vector<int> count(nb, 0);
   std::vector<float4> binfo(nb);
   for (int b = 0; b < nb; ++b)
@@ -618,6 +621,7 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
                            0.0f);
   for (int i = 0; i < n; ++i) {
     const int b = bin_of(h[i]);
+    bin[i] = static_cast<uint16_t>(b);
     ++count[b];
     binfo[b].z = std::max(binfo[b].z, std::fabs(cons[i].weight));
   }
@@ -630,7 +634,7 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   const int n_slots = std::max(start[nb], kBinPad);
   std::vector<int> map(n_slots, -1);
   std::vector<int> fill(start.begin(), start.end() - 1);
-  for (int i = 0; i < n; ++i) map[fill[bin_of(h[i])]++] = i;
+  for (int i = 0; i < n; ++i) map[fill[bin[i]]++] = i;
   const int steps = n_slots / kStep;
   std::vector<uint16_t> sbin(steps, 0);
   for (int b = 0; b < nb; ++b)
@@ -656,7 +660,7 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   for (int k = 0; k < n_slots; ++k) {
     const int i = map[k];
     if (i < 0) continue;
-    const int b = bin_of(h[i]);
+    const int b = bin[i];
     const double* nd = &ndd[static_cast<size_t>(b) * kMaxNodes];
     for (int t = 0; t < T; ++t) {
       double l = inv[static_cast<size_t>(b) * kMaxNodes + t] * binfo[b].w;
