@@ -78,6 +78,12 @@ int mfma_kind(const GroupInfo& info, int basis);
 // Drops what the MFMA forms cached about a group (called on unregister).
 void mfma_forget_group(const void* key);
 
+// At registration (host arrays at hand): builds the group's h-bins for the config's MFMA setting
+// from the host arrays, with the table uploads on stream s, so that the first render finds them
+// (no copy back from the device).  Nothing for the plain form or a uniform radius; false (error
+// recorded) on failure.
+bool mfma_prebin(const Group& g, const GroupInfo& info, const VisConfig* c, hipStream_t s);
+
 }  // namespace mv
 
 #include "mfma_plan.h"

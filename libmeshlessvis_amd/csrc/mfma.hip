@@ -383,6 +383,42 @@ int mfma_kind(const GroupInfo& info, int basis) {
   return basis == GAUSSIAN ? KIND_GAUSS : KIND_BINNED;
 }
 
+// The h-bins (count nb) and interpolation nodes (T) a binned kind takes under a config; false
+// (error recorded) on an invalid setting.
+static bool bin_setting(const GroupInfo& info, int kind, const VisConfig* c, int* nb_out, int* T_out) {
+  int nb, T = 1;
+  if (kind == KIND_GAUSS) {
+    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 1.25;
+    nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
+  } else {
+    T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 2;
+    if (T < 2 || T > kMaxNodes) {
+      set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
+      return false;
+    }
+    // Defaults: 2 Chebyshev nodes per bin of ratio 1.01 (the two-node kernel; interpolation
+    // error < 1.2e-5 of Phi(0) pointwise), 3 nodes: 1.04 (< 3e-6), more: 1.1.
+    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? 1.01 : T == 3 ? 1.04 : 1.1);
+    nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
+  }
+  if (nb > 65535) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
+    return false;
+  }
+  *nb_out = nb;
+  *T_out = T;
+  return true;
+}
+
+bool mfma_prebin(const Group& g, const GroupInfo& info, const VisConfig* c, hipStream_t s) {
+  if (c->form == VIS_FORM_PLAIN || !g.d_constraints || !g.h_radii || !g.h_constraints || info.n <= 0) return true;
+  const int kind = mfma_kind(info, g.basis_function_id);
+  if (kind == KIND_UNIFORM) return true;
+  int nb, T;
+  if (!bin_setting(info, kind, c, &nb, &T)) return false;
+  return bin_cache_from_host(g, info, info.n, nb, T, s) != nullptr;
+}
+
 // Bins, packet slots and work items of one job (host side; builds the group's
 // bins on first use).
 bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, MfmaPlanned* out) {
@@ -402,25 +438,8 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
     job.bin0 = make_float4(0.0f, 0.0f, info.w_absmax, info.w_absmax > 0.0f ? 1.0f / info.w_absmax : 0.0f);
     bin_step = {0, out->n_slots / kStep};
   } else {
-    int nb, T = 1;
-    if (out->kind == KIND_GAUSS) {
-      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 1.25;
-      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
-    } else {
-      T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 2;
-      if (T < 2 || T > kMaxNodes) {
-        set_error(VIS_ERROR_INVALID_CONFIG, "number_of_interpolation_nodes must be 0 (auto) or 2..5");
-        return false;
-      }
-      // Defaults: 2 Chebyshev nodes per bin of ratio 1.01 (the two-node kernel; interpolation
-      // error < 1.2e-5 of Phi(0) pointwise), 3 nodes: 1.04 (< 3e-6), more: 1.1.
-      const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? 1.01 : T == 3 ? 1.04 : 1.1);
-      nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
-    }
-    if (nb > 65535) {
-      set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
-      return false;
-    }
+    int nb, T;
+    if (!bin_setting(info, out->kind, c, &nb, &T)) return false;
     const BinCache* bc = bin_cache(*spec.g, info, spec.n, nb, T, s);
     if (!bc) return false;
     out->bc = bc;

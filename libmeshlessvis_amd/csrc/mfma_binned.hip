@@ -590,16 +590,10 @@ int auto_bins(float h_min, float h_max, double ratio) {
 // Sort the group's particles into nb bins of h (log-spaced when h_min > 0),
 // each padded to whole 16-particle steps, with T Chebyshev nodes per bin and
 // the bin's largest |w|.
-static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s, BinCache* bc) {
-  std::vector<float> h(n);
-  std::vector<Constraint> cons(n);
-  if (n > 0) {
-    if (!hip_ok(hipMemcpyAsync(h.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s), "hipMemcpy(radii)") ||
-        !hip_ok(hipMemcpyAsync(cons.data(), g.d_constraints, sizeof(Constraint) * n, hipMemcpyDeviceToHost, s),
-                "hipMemcpy(constraints)"))
-      return false;
-    if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
-  }
+// From host arrays h[n] (radii) and cons[n]; the tables are uploaded on stream s from host
+// copies kept in bc.
+static bool build_bins_host(const float* h, const Constraint* cons, const GroupInfo& info, int n, int nb, int T,
+                            hipStream_t s, BinCache* bc) {
   const double lo = info.h_min, hi = info.h_max;
   const bool logb = lo > 0.0 && hi > lo;
   std::vector<double> edge(nb + 1);
@@ -615,7 +609,8 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   // Each particle's bin, once (one logarithm per particle).
   std::vector<uint16_t> bin(n);
   std::vector<int> count(nb, 0);
-  std::vector<float4> binfo(nb);
+  std::vector<float4>& binfo = bc->h_binfo;
+  binfo.assign(nb, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   for (int b = 0; b < nb; ++b)
     binfo[b] = make_float4(static_cast<float>(edge[b] * edge[b]), static_cast<float>(edge[b + 1] * edge[b + 1]), 0.0f,
                            0.0f);
@@ -632,14 +627,17 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   std::vector<int> start(nb + 1, 0);
   for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kBinPad - 1) / kBinPad) * kBinPad;
   const int n_slots = std::max(start[nb], kBinPad);
-  std::vector<int> map(n_slots, -1);
+  std::vector<int>& map = bc->h_map;
+  map.assign(n_slots, -1);
   std::vector<int> fill(start.begin(), start.end() - 1);
   for (int i = 0; i < n; ++i) map[fill[bin[i]]++] = i;
   const int steps = n_slots / kStep;
-  std::vector<uint16_t> sbin(steps, 0);
+  std::vector<uint16_t>& sbin = bc->h_sbin;
+  sbin.assign(steps, 0);
   for (int b = 0; b < nb; ++b)
     for (int k = start[b] / kStep; k < start[b + 1] / kStep; ++k) sbin[k] = static_cast<uint16_t>(b);
-  std::vector<float> nodes(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
+  std::vector<float>& nodes = bc->h_nodes;
+  nodes.assign(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
   std::vector<double> ndd(static_cast<size_t>(nb) * kMaxNodes, 0.0), inv(static_cast<size_t>(nb) * kMaxNodes, 0.0);
   for (int b = 0; b < nb; ++b) {
     const double c = 0.5 * (edge[b] + edge[b + 1]), d = 0.5 * (edge[b + 1] - edge[b]);
@@ -656,7 +654,8 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
   // Lagrange weights of every particle at its bin's nodes, scaled by 1 / max |w|
   // of the bin (view-independent, so computed once here).
   const int lw = 4 * lag_words(T);
-  std::vector<float> lag(static_cast<size_t>(n_slots) * lw, 0.0f);
+  std::vector<float>& lag = bc->h_lag;
+  lag.assign(static_cast<size_t>(n_slots) * lw, 0.0f);
   for (int k = 0; k < n_slots; ++k) {
     const int i = map[k];
     if (i < 0) continue;
@@ -679,26 +678,57 @@ static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int
       !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->lag.reserve(sizeof(float) * lag.size()) ||
       !bc->bininfo.reserve(sizeof(float4) * nb))
     return false;
-  return hip_ok(hipMemcpy(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots, hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps, hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size(), hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->lag.ptr, lag.data(), sizeof(float) * lag.size(), hipMemcpyHostToDevice), "upload") &&
-         hip_ok(hipMemcpy(bc->bininfo.ptr, binfo.data(), sizeof(float4) * nb, hipMemcpyHostToDevice), "upload");
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "upload bins");
+  };
+  return up(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots) &&
+         up(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps) &&
+         up(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size()) &&
+         up(bc->lag.ptr, lag.data(), sizeof(float) * lag.size()) &&
+         up(bc->bininfo.ptr, binfo.data(), sizeof(float4) * nb);
 }
 
-const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s) {
+// From a copy of the group's device arrays (the render's stream s orders it after the
+// registration's copies).  Returns with the tables on the device: a cached binning may be used
+// next from another config's stream.
+static bool build_bins(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s, BinCache* bc) {
+  std::vector<float> h(n);
+  std::vector<Constraint> cons(n);
+  if (n > 0) {
+    if (!hip_ok(hipMemcpyAsync(h.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s), "hipMemcpy(radii)") ||
+        !hip_ok(hipMemcpyAsync(cons.data(), g.d_constraints, sizeof(Constraint) * n, hipMemcpyDeviceToHost, s),
+                "hipMemcpy(constraints)"))
+      return false;
+    if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
+  }
+  return build_bins_host(h.data(), cons.data(), info, n, nb, T, s, bc) &&
+         hip_ok(hipStreamSynchronize(s), "synchronize bins");
+}
+
+static const BinCache* bin_cache_impl(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s,
+                                      bool from_host) {
   std::lock_guard<std::mutex> lk(g_bin_mu);
   auto key = std::make_tuple(static_cast<const void*>(g.d_constraints), nb, T);
   BinCache& e = bin_caches()[key];
   if (e.n != n || !e.slot_map.ptr) {
     e.release();
-    if (!build_bins(g, info, n, nb, T, s, &e)) {
+    const bool ok = from_host ? build_bins_host(g.h_radii, g.h_constraints, info, n, nb, T, s, &e)
+                              : build_bins(g, info, n, nb, T, s, &e);
+    if (!ok) {
       e.release();
       bin_caches().erase(key);
       return nullptr;
     }
   }
   return &e;
+}
+
+const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s) {
+  return bin_cache_impl(g, info, n, nb, T, s, false);
+}
+
+const BinCache* bin_cache_from_host(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s) {
+  return bin_cache_impl(g, info, n, nb, T, s, true);
 }
 
 template <int OP, bool SPLIT, int T, int BASIS, int MODE>

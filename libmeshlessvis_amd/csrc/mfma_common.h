@@ -201,16 +201,29 @@ struct BinCache {
   DevBuf lag;       // float4[n_slots][lag_words(T)]: Lagrange weights l_t(h_i) / max |w| of the bin, 0 on padding
   DevBuf bininfo;   // float4[nb] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
   std::vector<int> bin_step;  // host: first step of every bin, nb + 1 entries
+  // Host copies of the tables: the sources of their stream-ordered uploads, kept until release.
+  std::vector<int> h_map;
+  std::vector<uint16_t> h_sbin;
+  std::vector<float> h_nodes, h_lag;
+  std::vector<float4> h_binfo;
   void release() {
-    slot_map.release();
+    slot_map.release();  // hipFree waits for the device, so the uploads above are done
     step_bin.release();
     nodes.release();
     lag.release();
     bininfo.release();
+    h_map.clear();
+    h_sbin.clear();
+    h_nodes.clear();
+    h_lag.clear();
+    h_binfo.clear();
   }
 };
-// The group's binning for (nb, T), built on first use; nullptr (error recorded) on failure.
+// The group's binning for (nb, T), built on first use (from a copy of the device arrays);
+// nullptr (error recorded) on failure.
 const BinCache* bin_cache(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s);
+// The same, built from the group's host arrays (which must hold what was registered).
+const BinCache* bin_cache_from_host(const Group& g, const GroupInfo& info, int n, int nb, int T, hipStream_t s);
 // Bins of ratio `ratio` over [h_min, h_max], at least 1, at most 1024.
 int auto_bins(float h_min, float h_max, double ratio);
 

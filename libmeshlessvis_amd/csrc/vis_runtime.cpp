@@ -753,7 +753,13 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
       info.uniform_radius = true;
       info.radius = 1.0f;
     }
-    if (g.d_constraints) infos.emplace_back(g.d_constraints, info);
+    if (g.d_constraints) {
+      infos.emplace_back(g.d_constraints, info);
+      // The config's MFMA form bins the group from the host arrays now, on the same stream as
+      // the copies: the first render needs no copy back from the device, and with asynchronous
+      // registration this host work overlaps the work already queued.
+      if (!mfma_prebin(g, info, c, s)) return;
+    }
   }
   if (async) {
     hipEvent_t ev = nullptr;
