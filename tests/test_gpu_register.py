@@ -57,3 +57,30 @@ def test_async_registration_then_immediate_unregister():
     assert np.array_equal(a, cfg.image())
     cfg.unregister(ds)
     cfg.close()
+
+
+@pytest.mark.parametrize("basis", ["WENDLAND_D3_C2", "GAUSSIAN"])
+def test_bins_from_registration_match_bins_from_first_render(basis):
+    """A registration through an MFMA config bins the groups from the host arrays; one through a
+    plain config leaves the binning to the first MFMA render (from a device copy).  Both give the
+    same image bit for bit."""
+    rng = np.random.default_rng(9)
+    n = 20000
+    ds = host.Dataset([(rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32), rng.uniform(0, 1, n).astype(np.float32),
+                        rng.uniform(0.005, 0.05, n).astype(np.float32), basis)])
+    geo = dict(cutoff=(64, 48), samples=(128, 128), step=(0.5, 0.5), u=(0.6, 0.8, 0.0), v=(-0.48, 0.36, 0.8),
+               phi_mode="analytic")
+    mf = host.Config(form="mfma_split_f16", **geo)
+    pl = host.Config(form="plain", **geo)
+    mf.register(ds)  # bins built here
+    mf.render(ds)
+    a = mf.image()
+    mf.unregister(ds)
+    pl.register(ds)  # no bins here: the MFMA render below builds them
+    mf.render(ds)
+    b = mf.image()
+    pl.unregister(ds)
+    mf.close()
+    pl.close()
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
