@@ -200,7 +200,7 @@ def batched_bench(args) -> None:
         cfgs[0].synchronize()
 
     t = time.perf_counter()
-    run(max(1, args.warmup))
+    run(max(2, args.warmup))  # a batch runs without a graph the first time, is captured the second
     log(f"warmup (graph capture, bins) {time.perf_counter() - t:.2f} s")
     cfgs[0].reset_timing()
     cfgs[0].set_timing(True)
@@ -389,7 +389,7 @@ def main() -> None:
     batched = None
     if form != "plain" and len(cfgs) > 1 and not args.no_batched:
         # The same views as one batched call: one captured graph, one persistent launch over all views.
-        for _ in range(max(1, args.warmup)):
+        for _ in range(max(2, args.warmup)):  # the second call captures the batch's graph
             host.render_batched([ds] * len(cfgs), cfgs)
         for c in cfgs:
             c.synchronize()

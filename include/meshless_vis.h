@@ -163,8 +163,9 @@ void vis_register_meshless_dataset_async(VisConfig* vis_config, MeshlessDataset*
 /* Render many (dataset, config) pairs; same results as calling
    vis_fourier_volume_rendering on each pair.  When every config has its own
    entry and they share a stream, an MFMA form, the Phi mode, cutoff, samples,
-   steps and bin settings, the whole batch runs as one captured HIP graph (one
-   persistent MFMA launch per kernel variant), replayed while the batch is
+   steps and bin settings, the whole batch runs as one persistent MFMA launch per
+   kernel variant (with batched prep, reductions and C2R): the first time at once,
+   the second time captured into a HIP graph, replayed while the batch is
    unchanged; otherwise the renders run one after the other. */
 void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, VisConfig** vis_configs, int number_of_renders);
 

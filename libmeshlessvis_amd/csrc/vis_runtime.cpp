@@ -85,7 +85,8 @@ void DevBuf::release() {
 // A captured batch render (vis_fourier_volume_rendering_batched), replayed while
 // its signature (configs, datasets, registrations, views, buffers) is unchanged.
 struct BatchGraph {
-  std::vector<uint64_t> key;
+  std::vector<uint64_t> key;   // the captured batch
+  std::vector<uint64_t> seen;  // the last batch run without a graph
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipStream_t capture = nullptr;  // default priority
@@ -481,7 +482,9 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
   return true;
 }
 
-static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGraph* bg) {
+// Plans the batch and enqueues its launches: captured into bg's graph (capture = true), or run
+// at once on the configs' stream.  Either way the same launches in the same order.
+static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGraph* bg, bool capture) {
   bg->drop();
   hipStream_t s = cf[0]->_plan->stream;
   std::vector<MfmaJobSpec> specs, one;
@@ -516,35 +519,39 @@ static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
   if (same_grid &&
       !upload_io_tables(specs, bg->ws.batch, cf, n, render_of, bg, &pjobs, &pbegin, &outs, &parts, &total))
     return false;
-  if (!bg->capture && !hip_ok(hipStreamCreateWithFlags(&bg->capture, hipStreamNonBlocking), "hipStreamCreate"))
-    return false;
-  if (!hip_ok(hipStreamBeginCapture(bg->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
-    return false;
+  if (capture) {
+    if (!bg->capture && !hip_ok(hipStreamCreateWithFlags(&bg->capture, hipStreamNonBlocking), "hipStreamCreate"))
+      return false;
+    if (!hip_ok(hipStreamBeginCapture(bg->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture"))
+      return false;
+  }
+  const hipStream_t q = capture ? bg->capture : s;
   bool ok;
   if (same_grid) {
     // One launch each: every job's packets, the persistent MFMA kernels, every render's
     // reduction + finalize, and the two C2R passes of every render.
     const VisConfig* c0 = cf[0];
-    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, bg->capture), "prep launch") &&
-         mfma_launch_sums(specs, bg->ws.batch, bg->capture, nullptr) &&
-         hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, bg->capture),
+    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, q), "prep launch") &&
+         mfma_launch_sums(specs, bg->ws.batch, q, nullptr) &&
+         hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, q),
                 "finish launch") &&
          hip_ok(launch_c2r_batch(outs, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, c0->_number_of_samples.x,
-                                 c0->_number_of_samples.y, c0->_plan->tw_x, c0->_plan->tw_y, bg->capture),
+                                 c0->_number_of_samples.y, c0->_plan->tw_x, c0->_plan->tw_y, q),
                 "c2r launch");
   } else {
-    ok = mfma_enqueue(specs, bg->ws.batch, bg->capture, nullptr);
+    ok = mfma_enqueue(specs, bg->ws.batch, q, nullptr);
     for (int i = 0; ok && i < n; ++i) {
       VisConfig* c = cf[i];
       const int nfreq = static_cast<int>(spectrum_count(c));
-      ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, bg->capture),
+      ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, q),
                   "finalize launch") &&
            hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
                              c->_number_of_samples.y, c->_plan->tw_x, c->_plan->tw_y, c->_d_freq_image_arranged,
-                             c->_d_image, bg->capture),
+                             c->_d_image, q),
                   "c2r launch");
     }
   }
+  if (!capture) return ok;
   hipGraph_t graph = nullptr;
   const hipError_t end = hipStreamEndCapture(bg->capture, &graph);
   if (!ok) {
@@ -560,18 +567,28 @@ static bool capture_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
   return true;
 }
 
+// A batch seen for the first time runs at once, without a graph (a one-off batch, such as one
+// timestep of an animation, would not repay the capture and instantiation); the same batch again
+// is captured, and replayed while it stays the same.
 static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
   VisPlan* p0 = cf[0]->_plan;
   BatchGraph& bg = p0->batch;
   std::vector<uint64_t> key = batch_key(ds, cf, n);
+  EventPair* ev = p0->timing ? next_events(p0) : nullptr;
   if (!bg.exec || key != bg.key) {
-    if (!capture_batch(ds, cf, n, &bg)) {
+    if (key != bg.seen) {
+      if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
+      const bool ok = enqueue_batch(ds, cf, n, &bg, false);
+      bg.seen = ok ? key : std::vector<uint64_t>();
+      if (ok && ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
+      return;
+    }
+    if (!enqueue_batch(ds, cf, n, &bg, true)) {
       bg.drop();
       return;
     }
     bg.key = key;
   }
-  EventPair* ev = p0->timing ? next_events(p0) : nullptr;
   if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
   if (!hip_ok(hipGraphLaunch(bg.exec, p0->stream), "hipGraphLaunch")) return;
   if (ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
