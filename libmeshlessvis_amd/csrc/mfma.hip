@@ -566,15 +566,41 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   if (specs.empty()) return true;
   // Items per batch: about 16 per CU over the whole batch, so a batch of many small
   // renders is not split more finely than it needs to be.
+  // A plan does not depend on the view: jobs of one group under equal settings (the views of a
+  // timestep) share it, so each distinct one is planned once.
+  auto same_plan = [](const MfmaJobSpec& a, const MfmaJobSpec& b) {
+    return a.g->d_constraints == b.g->d_constraints && a.info.serial == b.info.serial && a.n == b.n &&
+           a.vp.cx == b.vp.cx && a.vp.cy == b.vp.cy && a.vp.basis == b.vp.basis &&
+           a.c->number_of_h_bins == b.c->number_of_h_bins &&
+           a.c->number_of_interpolation_nodes == b.c->number_of_interpolation_nodes &&
+           a.c->h_bin_ratio == b.c->h_bin_ratio;
+  };
+  std::vector<size_t> first(specs.size());
+  for (size_t k = 0; k < specs.size(); ++k) {
+    first[k] = k;
+    for (size_t j = 0; j < k; ++j)
+      if (first[j] == j && same_plan(specs[j], specs[k])) {
+        first[k] = j;
+        break;
+      }
+  }
   long long work = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
-    if (!mfma_plan(specs[k], s, 0, &bt->plans[k])) return false;
+    if (first[k] != k)
+      bt->plans[k] = bt->plans[first[k]];
+    else if (!mfma_plan(specs[k], s, 0, &bt->plans[k]))
+      return false;
     work += bt->plans[k].tile_steps;
   }
   const long long target = std::max<long long>(1, work / (16LL * device_cus()));
   size_t slots = 0, parts = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
-    if (specs.size() > 1 && !mfma_plan(specs[k], s, target, &bt->plans[k])) return false;
+    if (specs.size() > 1) {
+      if (first[k] != k)
+        bt->plans[k] = bt->plans[first[k]];
+      else if (!mfma_plan(specs[k], s, target, &bt->plans[k]))
+        return false;
+    }
     bt->plans[k].pk_off = slots;
     bt->plans[k].part_off = parts;
     slots += bt->plans[k].n_slots;
