@@ -171,6 +171,9 @@ def batched_bench(args) -> None:
                                     phi_mode=wl.phi_mode))
             pairs.append(ds)
             meta.append((wl, vw))
+    if args.h_bin_ratio > 0.0:
+        for c in cfgs:
+            c.cs.h_bin_ratio = args.h_bin_ratio
     for ds in sets:
         cfgs[0].register(ds)
     tuned = None
@@ -335,6 +338,8 @@ def main() -> None:
                     help="drifting_clusters: skip the per-timestep upload (registration) measurement")
     ap.add_argument("--autotune", type=float, default=0.0, metavar="TOL",
                     help="before timing, let vis_autotune pick nodes/bin ratio within TOL of the plain kernel")
+    ap.add_argument("--h-bin-ratio", type=float, default=0.0, metavar="R",
+                    help="ratio of the h-bins of the MFMA forms (0: the library's default for the kind)")
     ap.add_argument("--cull", action="store_true",
                     help="render with cull_fully_aliased_terms set (drop terms wholly outside the period window)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -366,6 +371,9 @@ def main() -> None:
     ds = host.Dataset([wl.group()])
     cfgs = [host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
                         phi_mode=wl.phi_mode, cull=args.cull) for vw in wl.views]
+    if args.h_bin_ratio > 0.0:
+        for c in cfgs:
+            c.cs.h_bin_ratio = args.h_bin_ratio
     cfgs[0].register(ds)
     tuned = None
     if args.autotune > 0 and form != "plain":
@@ -482,6 +490,7 @@ def main() -> None:
             "phi_mode": wl.phi_mode,
             "form": form,
             "autotune": tuned,
+            "h_bin_ratio": args.h_bin_ratio or None,
             "cull_fully_aliased_terms": bool(args.cull),
         },
         "views_per_s": views / elapsed,

@@ -11,7 +11,7 @@
 //      alpha = beta = 1, E = Phi(r h) * W
 //  * Gaussian with per-particle radii (exact, no interpolation: exp(s pi h^2 r^2)
 //    = exp(s pi h^2 fu^2) exp(s pi h^2 fv^2), s = -1 analytic, +1 reference),
-//    log-spaced h-bins of ratio 1.25:
+//    log-spaced h-bins of ratio 2 (default):
 //      alpha_i(x) = exp(s pi (h_i^2 - h_b^2) fu^2), beta_i(y) likewise with fv,
 //      E_b = exp(s pi h_b^2 r^2) * W_b, h_b the bin's lower (s=-1) or upper (s=+1)
 //      edge so that |alpha|, |beta| <= 1;
@@ -388,7 +388,10 @@ int mfma_kind(const GroupInfo& info, int basis) {
 static bool bin_setting(const GroupInfo& info, int kind, const VisConfig* c, int* nb_out, int* T_out) {
   int nb, T = 1;
   if (kind == KIND_GAUSS) {
-    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 1.25;
+    // Default ratio 2: the bins only keep alpha, beta <= 1 in FP16 range, and a factor small
+    // enough to leave it belongs to a negligible term; fewer bins mean fewer bin-aligned work
+    // items (the 4M-particle log-normal view: 19.8 ms at 1.25, 19.3 ms at 2, same accuracy).
+    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : 2.0;
     nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
   } else {
     T = c->number_of_interpolation_nodes > 0 ? c->number_of_interpolation_nodes : 2;
