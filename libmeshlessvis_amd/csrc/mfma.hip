@@ -477,10 +477,15 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
     // items leaves the other CUs idle (a plan of 2.03 rounds takes 3).  Few rounds: every
     // item writes a partial spectrum tile that the reduction reads back (the 1M-particle
     // uniform view: 8 rounds 1.72 ms per view, 1.41 of them in the MFMA kernel; 2 rounds
-    // 1.48 ms, 1.38 in the kernel).
+    // 1.48 ms, 1.38 in the kernel).  Binned kernels take one round: an item also opens and
+    // folds its bins, and their FP32 sums are folded per bin, so longer items cost no accuracy
+    // (the 262k clustered view: 2.81e13 -> 2.97e13 terms/s); the sum kernel keeps two, its FP32
+    // accumulations lasting a whole item (one round: the 1M view 1% faster, 2.2e-6 -> 3.2e-6).
     const int cus = device_cus();
     const double w = static_cast<double>(total_steps) * tiles;
-    const long long rounds = std::max(1LL, std::min(2LL, std::llround(w / (static_cast<double>(floor_steps) * cus))));
+    const long long max_rounds = binned ? 1 : 2;
+    const long long rounds =
+        std::max(1LL, std::min(max_rounds, std::llround(w / (static_cast<double>(floor_steps) * cus))));
     const long long chunks = std::max(1LL, (rounds * cus + tiles - 1) / tiles);
     target_steps = (total_steps + chunks - 1) / chunks;
   }
