@@ -25,7 +25,7 @@ LIB = PKG / "libmeshless_vis.so"
 CLI = PKG / "vis_noninteractive"
 ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
 
-SOURCES = ["vis_runtime.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
+SOURCES = ["vis_runtime.cpp", "errors.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
 
 
 def _hipcc() -> str:

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "../../include/meshless_vis.h"
+#include "errors.h"
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -15,8 +16,6 @@
 namespace mv {
 
 // ---------------------------------------------------------------- errors ---
-void set_error(int code, const std::string& msg);
-void clear_error();
 bool hip_ok(hipError_t e, const char* what);  // records VIS_ERROR_HIP / OOM on failure
 
 // ------------------------------------------------------- particle packets ---
@@ -37,13 +36,18 @@ struct GroupInfo {
   bool has_radii = false;
   bool uniform_radius = false;  // every radius equal (or no radii)
   float radius = 1.0f;          // the common radius when uniform_radius
+  // Range of |r0| over the nonzero radii (Phi is even in r0, so a radius bins by its
+  // magnitude); has_zero_h: some radius is 0 (those particles get a bin of their own).
   float h_min = 1.0f, h_max = 1.0f;
+  bool has_zero_h = false;
   float w_absmax = 0.0f;
   // Unique per registration (set by registry_put): device state derived from a
   // group (bins, captured graphs) is keyed by it, because a later registration can
   // reuse the same device addresses.
   unsigned long long serial = 0;
 };
+// The facts above from a group's host (or host-copied) arrays; radii may be nullptr.
+void group_stats(const Constraint* constraints, const float* radii, int n, GroupInfo* info);
 void registry_put(const void* key, const GroupInfo& info);
 bool registry_get(const void* key, GroupInfo* info);
 void registry_erase(const void* key);

@@ -9,7 +9,8 @@
 // then all positions, then all weights, then a radii flag and, if set, all
 // radii.  Deviation: the reference's writer takes the radii flag from dataset
 // 0 for every dataset (meshless.cpp:178); here each dataset writes its own.
-#include "internal.h"
+#include "../../include/meshless_vis.h"
+#include "errors.h"
 
 #include <fstream>
 #include <limits>
@@ -101,24 +102,46 @@ void load_meshless_datasets_from_file(const char* filename, MeshlessDataset** ou
     set_error(VIS_ERROR_INVALID_VALUE, std::string("cannot open ") + (filename ? filename : "(null)"));
     return;
   }
+  // Bytes left in the file bound every count read from it: a dataset takes at least one
+  // character, a group header or a term at least four tokens (seven characters).  A header
+  // claiming more than the file can hold is malformed and allocates nothing.
+  in.seekg(0, std::ios::end);
+  const long long file_bytes = static_cast<long long>(in.tellg());
+  in.seekg(0, std::ios::beg);
+  auto left = [&]() {
+    const long long at = static_cast<long long>(in.tellg());
+    return at < 0 ? 0LL : file_bytes - at;
+  };
   int nd = 0;
-  if (!(in >> nd) || nd < 0) {
+  if (!(in >> nd) || nd < 0 || nd > left()) {
     set_error(VIS_ERROR_INVALID_VALUE, "malformed dataset count");
     return;
   }
   if (nd == 0) return;
-  MeshlessDataset* ds = new MeshlessDataset[nd]();
+  // Counts come from the file: every array is allocated without throwing, and a count the
+  // host cannot hold is reported (VIS_ERROR_OUT_OF_MEMORY) instead of ending the process.
+  MeshlessDataset* ds = new (std::nothrow) MeshlessDataset[nd]();
+  if (!ds) {
+    set_error(VIS_ERROR_OUT_OF_MEMORY, "load_meshless_datasets_from_file: cannot allocate " + std::to_string(nd) +
+                                           " datasets");
+    return;
+  }
   int built = 0;
-  auto fail = [&](const std::string& why) {
+  auto fail_code = [&](int code, const std::string& why) {
     for (int i = 0; i < built; ++i) delete_meshless_dataset(ds[i]);
     delete[] ds;
-    set_error(VIS_ERROR_INVALID_VALUE, "malformed .sph file: " + why);
+    set_error(code, why);
+  };
+  auto fail = [&](const std::string& why) { fail_code(VIS_ERROR_INVALID_VALUE, "malformed .sph file: " + why); };
+  auto no_memory = [&](const std::string& what) {
+    fail_code(VIS_ERROR_OUT_OF_MEMORY, "load_meshless_datasets_from_file: cannot allocate " + what);
   };
   for (int i = 0; i < nd; ++i) {
     int ng = 0;
-    if (!(in >> ng) || ng < 0) return fail("group count");
+    if (!(in >> ng) || ng < 0 || 4LL * ng > left()) return fail("group count");
+    ds[i].groups = new (std::nothrow) Group[ng]();
+    if (!ds[i].groups) return no_memory(std::to_string(ng) + " groups");
     ds[i].number_of_groups = ng;
-    ds[i].groups = new Group[ng]();
     built = i + 1;
     for (int j = 0; j < ng; ++j) {
       std::string name, unused;
@@ -127,7 +150,9 @@ void load_meshless_datasets_from_file(const char* filename, MeshlessDataset** ou
       if (!(in >> g.number_of_terms >> name >> unused >> params) || g.number_of_terms < 0)
         return fail("group header");
       if (!basis_from_name(name, &g.basis_function_id)) return fail("unknown basis '" + name + "'");
-      g.h_constraints = new Constraint[g.number_of_terms]();
+      if (4LL * g.number_of_terms > left()) return fail("group header: more terms than the file holds");
+      g.h_constraints = new (std::nothrow) Constraint[g.number_of_terms]();
+      if (!g.h_constraints) return no_memory(std::to_string(g.number_of_terms) + " constraints");
     }
     for (int j = 0; j < ng; ++j) {
       Group& g = ds[i].groups[j];
@@ -145,7 +170,8 @@ void load_meshless_datasets_from_file(const char* filename, MeshlessDataset** ou
     for (int j = 0; j < ng; ++j) {
       Group& g = ds[i].groups[j];
       if (has_radii) {
-        g.h_radii = new float[g.number_of_terms]();
+        g.h_radii = new (std::nothrow) float[g.number_of_terms]();
+        if (!g.h_radii) return no_memory(std::to_string(g.number_of_terms) + " radii");
         for (int k = 0; k < g.number_of_terms; ++k)
           if (!(in >> g.h_radii[k])) return fail("radii");
       } else {

@@ -404,6 +404,9 @@ static bool bin_setting(const GroupInfo& info, int kind, const VisConfig* c, int
     const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? 1.01 : T == 3 ? 1.04 : 1.1);
     nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
   }
+  // Radii of 0 take a bin of their own (bin 0, width 0) ahead of the log-spaced bins of the
+  // nonzero radii.
+  if (info.has_zero_h) ++nb;
   if (nb > 65535) {
     set_error(VIS_ERROR_INVALID_CONFIG, "number_of_h_bins above 65535");
     return false;

@@ -587,24 +587,37 @@ int auto_bins(float h_min, float h_max, double ratio) {
   return static_cast<int>(std::min(1024.0, std::max(1.0, nb)));
 }
 
-// Sort the group's particles into nb bins of h (log-spaced when h_min > 0),
-// each padded to whole 16-particle steps, with T Chebyshev nodes per bin and
-// the bin's largest |w|.
+// Sort the group's particles into nb bins of |h| (Phi is even in h), each padded to whole
+// 16-particle steps, with T Chebyshev nodes per bin and the bin's largest |w|.  The nonzero
+// radii get log-spaced bins over [h_min, h_max] (linear if that range is empty); radii of 0,
+// when the group has any, get bin 0 of width 0, whose first node is 0 with Lagrange weight 1.
 // From host arrays h[n] (radii) and cons[n]; the tables are uploaded on stream s from host
 // copies kept in bc.
 static bool build_bins_host(const float* h, const Constraint* cons, const GroupInfo& info, int n, int nb, int T,
                             hipStream_t s, BinCache* bc) {
   const double lo = info.h_min, hi = info.h_max;
+  const int z = info.has_zero_h ? 1 : 0;  // bins ahead of the nonzero radii' bins
+  const int nbp = nb - z;                 // bins of the nonzero radii
+  if (nbp < 1) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "h-binning: no bin left for the nonzero radii");
+    return false;
+  }
   const bool logb = lo > 0.0 && hi > lo;
-  std::vector<double> edge(nb + 1);
-  for (int b = 0; b <= nb; ++b)
-    edge[b] = logb ? lo * std::pow(hi / lo, static_cast<double>(b) / nb) : lo + (hi - lo) * b / nb;
+  // Bin b spans [e_lo[b], e_hi[b]]; the zero bin is the point 0.
+  std::vector<double> e_lo(nb, 0.0), e_hi(nb, 0.0);
+  for (int b = 0; b < nbp; ++b) {
+    auto edge = [&](int k) { return logb ? lo * std::pow(hi / lo, static_cast<double>(k) / nbp) : lo + (hi - lo) * k / nbp; };
+    e_lo[z + b] = edge(b);
+    e_hi[z + b] = edge(b + 1);
+  }
   const double log_ratio = logb ? std::log(hi / lo) : 1.0;
   auto bin_of = [&](float x) {
-    if (!(hi > lo)) return 0;
-    const double f = logb ? std::log(x / lo) / log_ratio : (x - lo) / (hi - lo);
-    const int b = static_cast<int>(f * nb);
-    return std::min(nb - 1, std::max(0, b));
+    const double a = std::fabs(static_cast<double>(x));
+    if (z && !(a > 0.0)) return 0;
+    if (!(hi > lo)) return z;
+    const double f = logb ? std::log(a / lo) / log_ratio : (a - lo) / (hi - lo);
+    const int b = static_cast<int>(f * nbp);
+    return z + std::min(nbp - 1, std::max(0, b));
   };
   // Each particle's bin, once (one logarithm per particle).
   std::vector<uint16_t> bin(n);
@@ -612,7 +625,7 @@ static bool build_bins_host(const float* h, const Constraint* cons, const GroupI
   std::vector<float4>& binfo = bc->h_binfo;
   binfo.assign(nb, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
   for (int b = 0; b < nb; ++b)
-    binfo[b] = make_float4(static_cast<float>(edge[b] * edge[b]), static_cast<float>(edge[b + 1] * edge[b + 1]), 0.0f,
+    binfo[b] = make_float4(static_cast<float>(e_lo[b] * e_lo[b]), static_cast<float>(e_hi[b] * e_hi[b]), 0.0f,
                            0.0f);
   for (int i = 0; i < n; ++i) {
     const int b = bin_of(h[i]);
@@ -639,16 +652,18 @@ static bool build_bins_host(const float* h, const Constraint* cons, const GroupI
   std::vector<float>& nodes = bc->h_nodes;
   nodes.assign(static_cast<size_t>(nb) * kMaxNodes, 0.0f);
   std::vector<double> ndd(static_cast<size_t>(nb) * kMaxNodes, 0.0), inv(static_cast<size_t>(nb) * kMaxNodes, 0.0);
+  std::vector<char> point(nb, 0);  // bins of width 0: every particle sits on the first node
   for (int b = 0; b < nb; ++b) {
-    const double c = 0.5 * (edge[b] + edge[b + 1]), d = 0.5 * (edge[b + 1] - edge[b]);
+    const double c = 0.5 * (e_lo[b] + e_hi[b]), d = 0.5 * (e_hi[b] - e_lo[b]);
     double* nd = &ndd[static_cast<size_t>(b) * kMaxNodes];
-    for (int t = 0; t < T; ++t) nd[t] = c + d * std::cos((2.0 * t + 1.0) * M_PI / (2.0 * T));
+    point[b] = !(d > 0.0);
+    for (int t = 0; t < T; ++t) nd[t] = point[b] ? c : c + d * std::cos((2.0 * t + 1.0) * M_PI / (2.0 * T));
     for (int t = 0; t < T; ++t) {
       double den = 1.0;
       for (int q = 0; q < T; ++q)
         if (q != t) den *= (nd[t] - nd[q]);
       nodes[b * kMaxNodes + t] = static_cast<float>(nd[t]);
-      inv[b * kMaxNodes + t] = 1.0 / den;
+      inv[b * kMaxNodes + t] = point[b] ? 0.0 : 1.0 / den;
     }
   }
   // Lagrange weights of every particle at its bin's nodes, scaled by 1 / max |w|
@@ -661,10 +676,16 @@ static bool build_bins_host(const float* h, const Constraint* cons, const GroupI
     if (i < 0) continue;
     const int b = bin[i];
     const double* nd = &ndd[static_cast<size_t>(b) * kMaxNodes];
+    const double a = std::fabs(static_cast<double>(h[i]));
     for (int t = 0; t < T; ++t) {
-      double l = inv[static_cast<size_t>(b) * kMaxNodes + t] * binfo[b].w;
-      for (int q = 0; q < T; ++q)
-        if (q != t) l *= (static_cast<double>(h[i]) - nd[q]);
+      double l;
+      if (point[b]) {
+        l = t == 0 ? binfo[b].w : 0.0;
+      } else {
+        l = inv[static_cast<size_t>(b) * kMaxNodes + t] * binfo[b].w;
+        for (int q = 0; q < T; ++q)
+          if (q != t) l *= (a - nd[q]);
+      }
       lag[static_cast<size_t>(k) * lw + t] = static_cast<float>(l);
     }
   }

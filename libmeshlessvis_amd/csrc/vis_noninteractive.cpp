@@ -173,8 +173,15 @@ bool fail(const char* what) {
 
 bool convert_raw(const Options& o) {
   const size_t n = static_cast<size_t>(o.raw_nx) * o.raw_ny;
-  std::vector<float> img(n);
   FILE* f = std::fopen(o.raw.c_str(), "rb");
+  // The file must hold the whole image before anything is allocated for it.
+  long bytes = -1;
+  if (f && std::fseek(f, 0, SEEK_END) == 0) bytes = std::ftell(f);
+  if (f && (bytes < 0 || static_cast<unsigned long>(bytes) / sizeof(float) < n || std::fseek(f, 0, SEEK_SET) != 0)) {
+    std::fclose(f);
+    f = nullptr;
+  }
+  std::vector<float> img(f ? n : 0);
   if (!f || std::fread(img.data(), sizeof(float), n, f) != n) {
     if (f) std::fclose(f);
     std::fprintf(stderr, "vis_noninteractive: cannot read %zu floats from %s\n", n, o.raw.c_str());
@@ -232,12 +239,14 @@ bool render_all(const Options& o) {
       ok = false;
       break;
     }
+    // Checked before the unregister, which clears the last error.
     vis_copy_to_host(c, img.data());
-    vis_unregister_meshless_dataset(c, &ds[k]);
     if (vis_get_last_error_code() != 0) {
       ok = fail("vis_copy_to_host");
+      vis_unregister_meshless_dataset(c, &ds[k]);
       break;
     }
+    vis_unregister_meshless_dataset(c, &ds[k]);
     ok = write_pgm(name, img.data(), w, h, o.depth);
   }
   vis_config_destroy(c);

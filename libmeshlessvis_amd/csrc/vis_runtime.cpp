@@ -12,23 +12,14 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <atomic>
 #include <cstring>
+#include <limits>
 #include <unordered_map>
 
 namespace mv {
 
 // ---------------------------------------------------------------- errors ---
-static thread_local int g_err_code = VIS_SUCCESS;
-static thread_local std::string g_err_msg;
-
-void set_error(int code, const std::string& msg) {
-  g_err_code = code;
-  g_err_msg = msg;
-}
-void clear_error() {
-  g_err_code = VIS_SUCCESS;
-  g_err_msg.clear();
-}
 bool hip_ok(hipError_t e, const char* what) {
   if (e == hipSuccess) return true;
   const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? VIS_ERROR_OUT_OF_MEMORY : VIS_ERROR_HIP;
@@ -43,6 +34,37 @@ static std::unordered_map<const void*, GroupInfo>& reg() {
   static std::unordered_map<const void*, GroupInfo> m;
   return m;
 }
+void group_stats(const Constraint* constraints, const float* radii, int n, GroupInfo* info) {
+  float wmax = 0.0f;
+  for (int i = 0; constraints && i < n; ++i) wmax = std::max(wmax, std::fabs(constraints[i].weight));
+  info->w_absmax = wmax;
+  info->has_zero_h = false;
+  if (radii && n > 0) {
+    info->has_radii = true;
+    float lo = radii[0], hi = radii[0];
+    float amin = std::numeric_limits<float>::infinity(), amax = 0.0f;
+    for (int i = 0; i < n; ++i) {
+      lo = std::min(lo, radii[i]);
+      hi = std::max(hi, radii[i]);
+      const float a = std::fabs(radii[i]);
+      if (a > 0.0f)
+        amin = std::min(amin, a);
+      else
+        info->has_zero_h = true;
+      amax = std::max(amax, a);
+    }
+    info->uniform_radius = (lo == hi);
+    info->radius = lo;
+    info->h_min = std::isfinite(amin) ? amin : 0.0f;
+    info->h_max = amax;
+  } else {
+    info->has_radii = false;
+    info->uniform_radius = true;
+    info->radius = 1.0f;
+    info->h_min = info->h_max = 1.0f;
+  }
+}
+
 void registry_put(const void* key, const GroupInfo& info) {
   static unsigned long long next_serial = 0;
   std::lock_guard<std::mutex> lk(g_reg_mu);
@@ -263,14 +285,36 @@ static int choose_ksplit(int ntiles, int freq_tiles) {
   return std::max(1, k);
 }
 
-static bool group_info_for(const Group& g, GroupInfo* info) {
+// A group whose device arrays the caller filled itself (not through
+// vis_register_meshless_dataset, as the reference allows: the kernels read only
+// d_constraints, d_radii and the counts) has no registry entry.  Its facts (weight and
+// radius ranges, which the MFMA forms' scaling and bins need) come from a copy of those
+// arrays taken at every render, since the caller may rewrite them in place between
+// renders; each such copy gets a serial of its own, and the MFMA forms' cached bins of the
+// arrays are dropped so that they are rebuilt from the same data.
+static std::atomic<unsigned long long> g_transient_serial{1ull << 62};
+static bool group_info_for(const Group& g, hipStream_t s, GroupInfo* info) {
   if (registry_get(g.d_constraints, info)) return true;
-  // Registered by other means: assume per-particle radii when present.
-  info->n = g.number_of_terms;
+  *info = GroupInfo{};
+  int n = std::max(0, g.number_of_terms);
+  if (g.d_number_of_terms > 0) n = std::min(n, g.d_number_of_terms);
+  info->n = n;
   info->padded = g.d_number_of_terms;
-  info->has_radii = g.d_radii != nullptr;
-  info->uniform_radius = g.d_radii == nullptr;
-  info->radius = 1.0f;
+  std::vector<Constraint> hc(n);
+  std::vector<float> hr(g.d_radii ? n : 0);
+  if (n > 0 && g.d_constraints) {
+    if (!hip_ok(hipMemcpyAsync(hc.data(), g.d_constraints, sizeof(Constraint) * n, hipMemcpyDeviceToHost, s),
+                "hipMemcpy(d_constraints)") ||
+        (g.d_radii && !hip_ok(hipMemcpyAsync(hr.data(), g.d_radii, sizeof(float) * n, hipMemcpyDeviceToHost, s),
+                              "hipMemcpy(d_radii)")) ||
+        !hip_ok(hipStreamSynchronize(s), "synchronize"))
+      return false;
+  }
+  group_stats(hc.data(), g.d_radii ? hr.data() : nullptr, n, info);
+  info->serial = ++g_transient_serial;
+  // Renders on other streams may still read the bins about to be dropped.
+  if (g.d_constraints && !(hip_ok(hipDeviceSynchronize(), "synchronize"))) return false;
+  if (g.d_constraints) mfma_forget_group(g.d_constraints);
   return true;
 }
 
@@ -293,7 +337,7 @@ static bool collect_groups(MeshlessDataset* ds, VisConfig* c, double2* acc, std:
     }
     MfmaJobSpec sp;
     sp.g = &g;
-    group_info_for(g, &sp.info);
+    if (!group_info_for(g, c->_plan->stream, &sp.info)) return false;
     sp.n = std::max(0, std::min(sp.info.n, g.d_number_of_terms > 0 ? g.d_number_of_terms : sp.info.n));
     sp.vp = vp;
     sp.vp.basis = g.basis_function_id;
@@ -374,6 +418,13 @@ static bool batch_fusable(MeshlessDataset** ds, VisConfig** cf, int n) {
         c->number_of_interpolation_nodes != c0->number_of_interpolation_nodes ||
         c->h_bin_ratio != c0->h_bin_ratio || ds[i]->number_of_groups < 1)
       return false;
+    // A group filled by the caller (no registry entry) is re-read at every render; a
+    // captured graph could not follow it, so such a batch renders view by view.
+    for (int g = 0; g < ds[i]->number_of_groups; ++g) {
+      GroupInfo info;
+      const Group& gr = ds[i]->groups[g];
+      if (gr.d_constraints && !registry_get(gr.d_constraints, &info)) return false;
+    }
     for (int j = 0; j < i; ++j)
       if (cf[j] == c) return false;  // one render per config (each owns its output)
   }
@@ -466,6 +517,10 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
                b_jp = sizeof(JobPart) * nj;
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
   const size_t o_bg = up(b_pj), o_ro = o_bg + up(b_bg), o_jp = o_ro + up(b_ro), all = o_jp + up(b_jp);
+  // The tables of the previous batch (an eager run or a replay of its graph, queued on the
+  // configs' stream, possibly a non-blocking one) may still be read: wait for them before the
+  // buffer is reused.
+  if (!hip_ok(hipStreamSynchronize(cf[0]->_plan->stream), "synchronize")) return false;
   if (!bg->io.reserve(all)) return false;
   std::vector<char> h(all, 0);
   std::memcpy(h.data(), pj.data(), b_pj);
@@ -739,6 +794,7 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
     GroupInfo info;
     info.n = n;
     info.padded = static_cast<int>(rounded);
+    group_stats(g.h_constraints, g.h_radii, n, &info);
     if (g.h_constraints && rounded > 0) {
       if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_constraints), sizeof(Constraint) * rounded), "hipMalloc(d_constraints)"))
         return;
@@ -746,29 +802,12 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
           !hip_ok(hipMemcpyAsync(g.d_constraints, g.h_constraints, sizeof(Constraint) * n, hipMemcpyHostToDevice, s),
                   "hipMemcpy(d_constraints)"))
         return;
-      float wmax = 0.0f;
-      for (int i = 0; i < n; ++i) wmax = std::max(wmax, std::fabs(g.h_constraints[i].weight));
-      info.w_absmax = wmax;
     }
     if (g.h_radii && rounded > 0) {
       if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_radii), sizeof(float) * rounded), "hipMalloc(d_radii)")) return;
       if (!hip_ok(hipMemsetAsync(g.d_radii, 0, sizeof(float) * rounded, s), "hipMemset") ||
           !hip_ok(hipMemcpyAsync(g.d_radii, g.h_radii, sizeof(float) * n, hipMemcpyHostToDevice, s), "hipMemcpy(d_radii)"))
         return;
-      info.has_radii = true;
-      float lo = n ? g.h_radii[0] : 1.0f, hi = lo;
-      for (int i = 1; i < n; ++i) {
-        lo = std::min(lo, g.h_radii[i]);
-        hi = std::max(hi, g.h_radii[i]);
-      }
-      info.h_min = lo;
-      info.h_max = hi;
-      info.uniform_radius = (lo == hi);
-      info.radius = lo;
-    } else {
-      info.has_radii = false;
-      info.uniform_radius = true;
-      info.radius = 1.0f;
     }
     if (g.d_constraints) {
       infos.emplace_back(g.d_constraints, info);
@@ -1055,9 +1094,5 @@ void vis_reset_kernel_timing(VisConfig* c) {
   (void)hipStreamSynchronize(c->_plan->stream);
   c->_plan->events_used = 0;
 }
-
-int vis_get_last_error_code(void) { return g_err_code; }
-const char* vis_get_last_error(void) { return g_err_msg.c_str(); }
-void vis_clear_last_error(void) { clear_error(); }
 
 }  // extern "C"

@@ -249,3 +249,69 @@ def render(positions, weights, radii=None, basis="GAUSSIAN", **config) -> np.nda
             return cfg.image()
         finally:
             cfg.unregister(ds)
+
+
+# ------------------------------------------------------------------ HIP runtime ---
+# Device memory and streams for callers that hand the library their own device
+# arrays (filled groups, a manual _d_image) or their own stream.
+_HIP = None
+
+
+def hip() -> C.CDLL:
+    """The HIP runtime the library is linked against (libamdhip64), with the few calls used here."""
+    global _HIP
+    if _HIP is None:
+        lib()  # loads libamdhip64 as the library's dependency
+        H = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        H.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        H.hipFree.argtypes = [C.c_void_p]
+        H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        H.hipStreamCreateWithFlags.argtypes = [C.POINTER(C.c_void_p), C.c_uint]
+        H.hipStreamDestroy.argtypes = [C.c_void_p]
+        H.hipStreamSynchronize.argtypes = [C.c_void_p]
+        for f in ("hipMalloc", "hipFree", "hipMemcpy", "hipStreamCreateWithFlags", "hipStreamDestroy",
+                  "hipStreamSynchronize", "hipDeviceSynchronize"):
+            getattr(H, f).restype = C.c_int
+        _HIP = H
+    return _HIP
+
+
+def _hip_check(err: int, what: str) -> None:
+    if err != 0:
+        raise RuntimeError(f"{what} failed with HIP error {err}")
+
+
+class DeviceBuffer:
+    """hipMalloc'd bytes, optionally filled from a numpy array (hipMemcpy host to device)."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, nbytes: int, init: Optional[np.ndarray] = None):
+        self.ptr = C.c_void_p()
+        self.nbytes = int(nbytes)
+        _hip_check(hip().hipMalloc(C.byref(self.ptr), max(1, self.nbytes)), "hipMalloc")
+        if init is not None:
+            self.upload(init)
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _hip_check(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy")
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        _hip_check(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H), "hipMemcpy")
+        return out
+
+    def free(self) -> None:
+        if self.ptr.value:
+            hip().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def create_stream(non_blocking: bool = True) -> C.c_void_p:
+    """A HIP stream of default priority (hipStreamNonBlocking when non_blocking)."""
+    s = C.c_void_p()
+    _hip_check(hip().hipStreamCreateWithFlags(C.byref(s), 1 if non_blocking else 0), "hipStreamCreateWithFlags")
+    return s

@@ -60,7 +60,8 @@ def _lib() -> C.CDLL:
         L.mvo_spectrum64.restype = None
         L.mvo_spectrum64.argtypes = [i, P(i), P(i), P(P(f)), P(P(f)), f, f, P(f), P(f), i, i, i, P(i), i, P(d), i]
         L.mvo_port_spectrum32.restype = None
-        L.mvo_port_spectrum32.argtypes = [i, P(i), P(i), P(i), P(P(f)), P(P(f)), f, f, P(f), P(f), i, i, P(f), i, i, i]
+        L.mvo_port_spectrum32.argtypes = [i, P(i), P(i), P(i), P(P(f)), P(P(f)), f, f, P(f), P(f), i, i, P(f), i, i, i,
+                                          i, P(i), i]
         L.mvo_arrange64.restype = None
         L.mvo_arrange64.argtypes = [P(d), i, i, i, i, P(d)]
         L.mvo_c2r64.restype = None
@@ -152,20 +153,27 @@ def padded_count(n: int, block_length: int, partial_sums: int) -> int:
     return m * (n // m + min(1, n % m))
 
 
-def port_spectrum32(groups, cutoff, step, u, v, block_length=256, partial_sums=1, row_stride=1, nthreads=0) -> np.ndarray:
-    """FP32 restatement of the reference's CPU direct sum; (cy, 2cx) complex64."""
+def port_spectrum32(groups, cutoff, step, u, v, block_length=256, partial_sums=1, row_stride=1, nthreads=0,
+                    phi_mode="reference", idx=None) -> np.ndarray:
+    """FP32 restatement of the reference's CPU direct sum; (cy, 2cx) complex64, or (len(idx),)
+    at the flat indices idx.  phi_mode "analytic" swaps the reference's Phi formulas for the exact
+    transforms (rounded to FP32), keeping the reference's loop, FP32 phase and FP32 sums."""
     G = _Groups(groups)
     cx, cy = int(cutoff[0]), int(cutoff[1])
     padded = np.array([padded_count(int(c), block_length, partial_sums) for c in G.counts], np.int32)
     u = np.asarray(u, np.float32)
     v = np.asarray(v, np.float32)
-    out = np.zeros(2 * max(2 * cx * cy, 1), np.float32)
+    idx_arr = None if idx is None else np.ascontiguousarray(np.asarray(idx, np.int32).reshape(-1))
+    n_out = 2 * cx * cy if idx_arr is None else idx_arr.size
+    out = np.zeros(2 * max(n_out, 1), np.float32)
     _lib().mvo_port_spectrum32(
         G.n, _ptr(G.counts, C.c_int), _ptr(padded, C.c_int), _ptr(G.bases, C.c_int), G.cons, G.radii,
         float(step[0]), float(step[1]), _ptr(u, C.c_float), _ptr(v, C.c_float), cx, cy,
-        _ptr(out, C.c_float), int(block_length), int(row_stride), int(nthreads),
+        _ptr(out, C.c_float), int(block_length), int(row_stride), int(nthreads), _mode(phi_mode),
+        _ptr(idx_arr, C.c_int) if idx_arr is not None else None, n_out,
     )
-    return out[: 4 * cx * cy].view(np.complex64).reshape(cy, 2 * cx)
+    z = out[: 2 * n_out].view(np.complex64)
+    return z.reshape(cy, 2 * cx) if idx_arr is None else z
 
 
 def arrange64(F, cx, cy, Nx, Ny) -> np.ndarray:
@@ -196,3 +204,24 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b, a.dtype)
     den = np.linalg.norm(b.ravel())
     return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
+
+
+def stratified_sample(cutoff, rng, low=8, n_random=512):
+    """Flat spectrum indices [y][x0] for checks at sizes where the whole-plane FP64 sum is out of
+    reach: every index with |x| <= low and y <= low (where the energy sits) plus n_random drawn
+    from the rest, and weights such that sum(weight * |z[idx]|^2) estimates the whole plane's."""
+    cx, cy = int(cutoff[0]), int(cutoff[1])
+    x0 = np.arange(2 * cx)
+    x = np.where(x0 > cx, x0 - 2 * cx, x0)
+    low_idx = np.array([y * 2 * cx + i for y in range(min(cy, low + 1)) for i in range(2 * cx) if abs(x[i]) <= low],
+                       dtype=np.int64)
+    rest = np.setdiff1d(np.arange(2 * cx * cy), low_idx)
+    pick = rng.choice(rest, size=min(n_random, rest.size), replace=False)
+    weight = np.concatenate([np.ones(low_idx.size), np.full(pick.size, rest.size / max(1, pick.size))])
+    return np.concatenate([low_idx, pick]), weight
+
+
+def sampled_rel_l2(F, F64_at, idx, weight) -> float:
+    """Relative L2 of a spectrum F (any shape, flat [y][x0] order) against FP64 values at idx."""
+    err = np.abs(np.asarray(F).reshape(-1)[idx].astype(np.complex128) - F64_at) ** 2
+    return float(np.sqrt((weight * err).sum() / (weight * np.abs(F64_at) ** 2).sum()))

@@ -227,23 +227,35 @@ void mvo_spectrum64(int ngroups, const int* counts, const int* bases, const floa
   }
 }
 
+/* Phi of the port: the reference's FP32 formulas (mode MVO_PHI_REFERENCE), or the exact
+   transform evaluated in FP64 and rounded to FP32 (MVO_PHI_ANALYTIC: the reference's loop,
+   phase and FP32 sums with the Phi of configurations whose reference formulas overflow). */
+static float port_phi(int basis, int mode, float r) {
+  return mode == MVO_PHI_REFERENCE ? mvo_phi32_port(basis, r) : (float)mvo_phi64(basis, MVO_PHI_ANALYTIC, (double)r);
+}
+
 /*
  * FP32 restatement of fourier_transform_cpu.cpp:73-134 for all groups
  * (first group assigns, later groups accumulate, :305-312).  Only rows
  * y % row_stride == 0 are computed when row_stride > 1 (timing samples); the
- * other rows are left untouched.  `padded` particles per group are summed
- * (the reference loops over d_number_of_terms, zero padding included); entries
- * beyond counts[g] are treated as zero particles.
+ * other rows are left untouched.  With idx (nidx entries), only the flat
+ * frequency indices idx[j] are computed, each into out[2j], out[2j+1] (the same
+ * per-frequency loop, for parity checks at sizes where whole rows cost too much).
+ * `padded` particles per group are summed (the reference loops over
+ * d_number_of_terms, zero padding included); entries beyond counts[g] are
+ * treated as zero particles.
  */
 void mvo_port_spectrum32(int ngroups, const int* counts, const int* padded, const int* bases,
                          const float* const* cons, const float* const* radii, float sx, float sy, const float* u,
                          const float* v, int cx, int cy, float* out, int block_length, int row_stride,
-                         int nthreads) {
+                         int nthreads, int mode, const int* idx, int nidx) {
   const int image_size = 2 * cx * cy;
+  const int n_out = idx ? nidx : image_size;
   const float scale = sx * sy;
   if (nthreads > 0) omp_set_num_threads(nthreads);
   if (block_length < 1) block_length = 1;
   if (row_stride < 1) row_stride = 1;
+  init_tables();
   for (int g = 0; g < ngroups; ++g) {
     const float* c = cons[g];
     const float* h = radii[g];
@@ -251,11 +263,12 @@ void mvo_port_spectrum32(int ngroups, const int* counts, const int* padded, cons
     const int n = counts[g];
     const int np = padded[g] > n ? padded[g] : n;
 #pragma omp parallel for schedule(dynamic, block_length)
-    for (int index = 0; index < image_size; index++) {
+    for (int j = 0; j < n_out; j++) {
+      const int index = idx ? idx[j] : j;
       int x = index % (2 * cx);
       if (x > cx) x = x - (2 * cx);
       const int y = (index % image_size) / (2 * cx);
-      if (y % row_stride) continue;
+      if (!idx && y % row_stride) continue;
       const float fu = sx * x, fv = sy * y;
       const float fx = fu * u[0] + fv * v[0], fy = fu * u[1] + fv * v[1], fz = fu * u[2] + fv * v[2];
       const float r = sqrtf((float)(fu * fu + fv * fv));
@@ -269,17 +282,17 @@ void mvo_port_spectrum32(int ngroups, const int* counts, const int* padded, cons
           w = c[4 * k + 3];
           r0 = h ? h[k] : 0.0f;
         }
-        const float term = h ? mvo_phi32_port(basis, r * r0) : mvo_phi32_port(basis, r);
+        const float term = h ? port_phi(basis, mode, r * r0) : port_phi(basis, mode, r);
         const float vv = TWO_PI_F * (fx * px + fy * py + fz * pz);
         sre += w * term * cosf(vv);
         sim += w * term * sinf(vv);
       }
       if (g == 0) {
-        out[2 * index] = sre * scale;
-        out[2 * index + 1] = -sim * scale;
+        out[2 * j] = sre * scale;
+        out[2 * j + 1] = -sim * scale;
       } else {
-        out[2 * index] += sre * scale;
-        out[2 * index + 1] += -sim * scale;
+        out[2 * j] += sre * scale;
+        out[2 * j + 1] += -sim * scale;
       }
     }
   }

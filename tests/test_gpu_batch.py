@@ -2,8 +2,11 @@
 against one-at-a-time renders, and against the FP64 oracle on a batch of drifting clusters.
 
 The graph is replayed several times with eager renders and synchronisations between the
-replays; every replay must reproduce the eager images (same kernels, same work split, same
-ordered reductions: identical up to FP32 rounding of the final store).
+replays.  A batch's first call runs its launches without a graph and the next call captures
+and replays them: every replay must equal that first call bit for bit (same kernels, same work
+split, same ordered reductions).  Against one-at-a-time renders the images agree to FP32
+rounding (<= 1e-6 relative): a batch cuts its jobs into work items for the whole batch, a
+single render for itself, so the FP32 partial sums end at other particles.
 """
 import numpy as np
 import pytest
@@ -47,11 +50,17 @@ def test_batched_graph_replays_match_eager():
         cfg.render(ds)
         eager.append(cfg.image())
     other = host.Config(cutoff=(32, 32), samples=(64, 64), step=(0.5, 0.5), form="plain", phi_mode="analytic")
-    for rep in range(3):
+    first = None
+    for rep in range(4):  # rep 0 without a graph, rep 1 captures, reps 2, 3 replay
         host.render_batched(pairs, cfgs)
         imgs = [cfg.image() for cfg in cfgs]
+        spectra = [cfg.spectrum() for cfg in cfgs]
+        if first is None:
+            first = (imgs, spectra)
         for a, b in zip(imgs, eager):
             assert oracle.rel_l2(a, b) <= 1e-6, rep
+        for a, b in zip(imgs + spectra, first[0] + first[1]):
+            assert np.array_equal(a, b), rep
         other.render(sets[2])  # eager work between replays
         other.synchronize()
     # A changed view is a new batch signature: the graph is rebuilt and stays right.

@@ -138,3 +138,31 @@ def test_port_row_subset_leaves_other_rows():
     part = oracle.port_spectrum32([g], (4, 8), (1, 1), (1, 0, 0), (0, 1, 0), row_stride=4)
     assert np.array_equal(part[::4], full[::4])
     assert not np.any(part[1::4])
+
+
+def test_port_analytic_phi_mode():
+    """The port with the exact transforms in place of the reference's fits: within the FP32 port's
+    own error of the FP64 analytic oracle, and measurably apart from the reference-formula port
+    (the Wendland fit below r = 0.24 deviates from the exact transform by up to 2.9e-4 of Phi(0))."""
+    rng = np.random.default_rng(8)
+    g = _cloud(rng, 3000, "WENDLAND_D3_C2")
+    u, v = (0.6, 0.8, 0.0), (-0.48, 0.36, 0.8)
+    cutoff, step = (16, 12), (0.5, 0.5)
+    F64 = oracle.spectrum64([g], cutoff, step, u, v, "analytic")
+    Fan = oracle.port_spectrum32([g], cutoff, step, u, v, phi_mode="analytic")
+    Fref = oracle.port_spectrum32([g], cutoff, step, u, v, phi_mode="reference")
+    e = oracle.rel_l2(Fan, F64)
+    assert 1e-8 <= e <= 2e-5, e
+    assert oracle.rel_l2(Fref, F64) > 10 * e
+
+
+def test_port_at_indices_equals_whole_plane():
+    """Frequencies computed one by one (idx) are the whole-plane port's values bit for bit,
+    for a first group and an accumulated second group."""
+    rng = np.random.default_rng(9)
+    groups = [_cloud(rng, 300, "GAUSSIAN"), _cloud(rng, 200, "SPH")]
+    u, v = (1, 0, 0), (0, 0.6, 0.8)
+    full = oracle.port_spectrum32(groups, (6, 5), (0.7, 0.9), u, v).reshape(-1)
+    idx = rng.choice(full.size, 17, replace=False)
+    part = oracle.port_spectrum32(groups, (6, 5), (0.7, 0.9), u, v, idx=idx)
+    assert np.array_equal(part, full[idx])
