@@ -101,10 +101,41 @@ def time_renders(cfgs, ds, steps, warmup):
     return elapsed, launches, kernel_ms
 
 
-def cpu_baseline(wl, threads: int):
-    """The FP32 restatement of the reference CPU direct sum, timed on a bounded sample of view 0."""
+def host_info() -> dict:
+    """The host the CPU baseline runs on: usable cores, CPU model, OMP_NUM_THREADS."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"nproc": nproc, "cpu_count": os.cpu_count(), "cpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def baseline_threads(info: dict) -> int:
+    """Every core the process may use (its affinity mask, nproc), within the host's CPU share
+    for this process when OMP_NUM_THREADS sets one."""
+    n = info["nproc"]
+    omp = info["OMP_NUM_THREADS"]
+    if omp and omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(wl, threads: int | None = None):
+    """The FP32 restatement of the reference CPU direct sum, timed on a bounded sample of view 0,
+    on the host's cores (baseline_threads) unless told otherwise."""
     import oracle
 
+    info = host_info()
+    threads = threads or baseline_threads(info)
     vw = wl.views[0]
     n_sub = min(wl.n, 262_144)
     cy = wl.cutoff[1]
@@ -119,6 +150,7 @@ def cpu_baseline(wl, threads: int):
         "value": terms / dt,
         "unit": "terms/s",
         "cores": threads,
+        "host": info,
         "kind": "port",
         "sample": f"view 0, {rows} of {cy} v-rows x {n_sub} of {wl.n} particles ({terms:.3e} terms, {dt:.1f} s)",
     }
@@ -246,9 +278,8 @@ def batched_bench(args) -> None:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wls[0], threads)
-        log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({cpu['sample']})")
+        cpu = cpu_baseline(wls[0])
+        log(f"cpu baseline: {cpu['value']:.3e} terms/s on {cpu['cores']} threads ({cpu['sample']})")
 
     for ds in sets:
         cfgs[0].unregister(ds)
@@ -426,6 +457,7 @@ def main() -> None:
             f"{form} is {value / p_value:.1f}x")
 
     rel = None
+    reference_parity = None
     if not args.no_accuracy:
         rng = np.random.default_rng(0)
         idx, weight = sample_indices(wl.cutoff, rng)
@@ -437,6 +469,17 @@ def main() -> None:
         rel = rel_l2_sampled(F, F64, idx, weight)
         log(f"rel-L2 vs FP64 oracle (stratified sample of {idx.size} frequencies): {rel:.3e} "
             f"({time.perf_counter() - t:.1f} s)")
+        # The reference's CPU direct sum (FP32 port: its loop, FP32 phase and sums) on the same
+        # inputs and frequencies; with the workload's Phi mode (analytic: exact transforms in
+        # place of the reference's formulas, whose exp(+pi r^2) overflows at these radii).
+        t = time.perf_counter()
+        Fport = oracle.port_spectrum32([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, phi_mode=wl.phi_mode,
+                                       idx=idx).astype(np.complex128)
+        e_ref = rel_l2_sampled(Fport, F64, np.arange(idx.size), weight)
+        e_par = rel_l2_sampled(F, Fport, idx, weight)
+        reference_parity = {"e_ref": e_ref, "e_build": rel, "e_par": e_par, "phi_mode": wl.phi_mode,
+                            "frequencies": int(idx.size), "holds": bool(e_par <= e_ref + rel + 1e-6)}
+        log(f"reference port: e_ref {e_ref:.3e} e_par {e_par:.3e} ({time.perf_counter() - t:.1f} s)")
 
     avg_ms = kernel_ms / max(1, launches)
     launch_terms = wl.terms_per_view
@@ -458,9 +501,8 @@ def main() -> None:
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(wl, threads)
-        log(f"cpu baseline: {cpu['value']:.3e} terms/s on {threads} threads ({cpu['sample']})")
+        cpu = cpu_baseline(wl)
+        log(f"cpu baseline: {cpu['value']:.3e} terms/s on {cpu['cores']} threads ({cpu['sample']})")
 
     for c in cfgs[1:]:
         c.close()
@@ -495,6 +537,7 @@ def main() -> None:
         },
         "views_per_s": views / elapsed,
         "rel_l2_vs_fp64": rel,
+        "reference_parity": reference_parity,
         "batched": batched,
         "plain_fp32": plain,
         "roofline": roofline,
