@@ -169,6 +169,21 @@ void vis_register_meshless_dataset_async(VisConfig* vis_config, MeshlessDataset*
    unchanged; otherwise the renders run one after the other. */
 void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, VisConfig** vis_configs, int number_of_renders);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid it): the
+   render of vis_fourier_volume_rendering for a config with form VIS_FORM_PLAIN,
+   with every scratch buffer (packets, split-K partials, FP64 accumulator) in a
+   static 64 MiB device array that each GPU the library runs on holds until the
+   process ends.  Inputs (registered datasets) and outputs (the config's buffers)
+   are the caller's as usual; the call allocates nothing.  A render needing more
+   scratch (16 B per particle of the largest group, plus 16 B per frequency and
+   16 B per frequency per split-K slot, at least one) is refused with
+   VIS_ERROR_INVALID_VALUE; other forms with VIS_ERROR_INVALID_CONFIG, groups not
+   registered through vis_register_meshless_dataset with VIS_ERROR_NOT_REGISTERED.
+   All calls on one GPU share its array: calls that overlap on one GPU (other
+   streams or threads) race. */
+void vis_fourier_volume_rendering_noalloc(MeshlessDataset* meshless_dataset, VisConfig* vis_config);
+
 /* Picks number_of_interpolation_nodes and h_bin_ratio for the MFMA form on this
    (registered) dataset: every candidate is timed on the config's stream and its
    spectrum compared with the plain FP32 kernel's; the fastest candidate whose

@@ -129,6 +129,7 @@ PROTOTYPES = {
     "vis_copy_spectrum_to_host": (None, [VisConfigP, C.POINTER(float2)]),
     "vis_register_meshless_dataset_async": (None, [VisConfigP, DatasetP]),
     "vis_fourier_volume_rendering_batched": (None, [C.POINTER(DatasetP), C.POINTER(VisConfigP), C.c_int]),
+    "vis_fourier_volume_rendering_noalloc": (None, [DatasetP, VisConfigP]),
     "vis_autotune": (C.c_double, [DatasetP, VisConfigP, C.c_double]),
     "vis_set_kernel_timing": (None, [VisConfigP, C.c_int]),
     "vis_get_kernel_timing": (C.c_int, [VisConfigP, C.POINTER(C.c_double)]),

@@ -649,6 +649,119 @@ static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
   if (ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
 }
 
+
+// ------------------------------------------------------- allocation-free render ---
+// vis_fourier_volume_rendering_noalloc: the plain-form render with every scratch buffer
+// (per-view packets, split-K FP64 partials, FP64 spectrum accumulator) in one static device
+// array per GPU, so the call itself allocates no device memory.  The array is reached through
+// its address from hipGetSymbolAddress, taken once per device with that device current (in
+// host code the array's name is not a device address).
+constexpr size_t kNoallocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+static unsigned char* scratch_for_current_device() {
+  int dev = 0;
+  if (!hip_ok(hipGetDevice(&dev), "hipGetDevice")) return nullptr;
+  if (dev < 0 || dev >= kMaxScratchDevices) {
+    set_error(VIS_ERROR_UNSUPPORTED, "vis_fourier_volume_rendering_noalloc: device index out of range");
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (!hip_ok(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)")) return nullptr;
+    g_scratch_ptr[dev] = ptr;
+  }
+  return static_cast<unsigned char*>(g_scratch_ptr[dev]);
+}
+
+static size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+static bool render_noalloc(MeshlessDataset* ds, VisConfig* c) {
+  VisPlan* p = c->_plan;
+  const int nfreq = static_cast<int>(spectrum_count(c));
+  const int ngroups = ds->number_of_groups;
+  if (ngroups < 1 || nfreq == 0) {
+    if (!hip_ok(launch_zero_words(c->_d_freq_image, 2 * static_cast<size_t>(nfreq), p->stream), "zero spectrum"))
+      return false;
+  } else {
+    // Sizes first: the largest group's packets, the largest split-K partials.
+    const int freq_tiles = ((2 * c->_cutoff_frequency.x + 63) / 64) * ((c->_cutoff_frequency.y + 31) / 32);
+    size_t max_padded = 0, max_ksplit = 1;
+    for (int gi = 0; gi < ngroups; ++gi) {
+      const Group& g = ds->groups[gi];
+      if (g.basis_function_id < SPH || g.basis_function_id > WENDLAND_D3_C2) {
+        set_error(VIS_ERROR_INVALID_VALUE, "unknown basis_function_id");
+        return false;
+      }
+      GroupInfo info;
+      if (g.number_of_terms > 0 && (!g.d_constraints || !registry_get(g.d_constraints, &info))) {
+        set_error(VIS_ERROR_NOT_REGISTERED,
+                  "vis_fourier_volume_rendering_noalloc: every group must be registered (vis_register_meshless_dataset)");
+        return false;
+      }
+      const size_t n = g.number_of_terms > 0 ? static_cast<size_t>(info.n) : 0;
+      const size_t padded = (n + 255) / 256 * 256;
+      max_padded = std::max(max_padded, padded);
+      max_ksplit = std::max<size_t>(max_ksplit, choose_ksplit(static_cast<int>(padded / 256), freq_tiles));
+    }
+    const size_t acc_b = align256(sizeof(double2) * nfreq);
+    const size_t pk_b = align256(sizeof(Packet) * std::max<size_t>(max_padded, 256));
+    const size_t part_one = sizeof(double2) * static_cast<size_t>(nfreq);
+    if (acc_b + pk_b + part_one > kNoallocScratchBytes) {
+      set_error(VIS_ERROR_INVALID_VALUE,
+                "vis_fourier_volume_rendering_noalloc: particles and spectrum need more than its 64 MiB scratch "
+                "(16 B per particle of the largest group + 32 B per frequency at least)");
+      return false;
+    }
+    // As many split-K slots as the render would take, fewer when the scratch holds fewer.
+    const size_t ksplit_cap = (kNoallocScratchBytes - acc_b - pk_b) / part_one;
+    unsigned char* base = scratch_for_current_device();
+    if (!base) return false;
+    double2* acc = reinterpret_cast<double2*>(base);
+    Packet* packets = reinterpret_cast<Packet*>(base + acc_b);
+    double2* partial = reinterpret_cast<double2*>(base + acc_b + pk_b);
+    ViewParams vp;
+    vp.cx = c->_cutoff_frequency.x;
+    vp.cy = c->_cutoff_frequency.y;
+    vp.sx = c->step_size.x;
+    vp.sy = c->step_size.y;
+    vp.phi_mode = c->phi_mode;
+    const double3 u = make_double3(c->u_axis.x, c->u_axis.y, c->u_axis.z);
+    const double3 v = make_double3(c->v_axis.x, c->v_axis.y, c->v_axis.z);
+    for (int gi = 0; gi < ngroups; ++gi) {
+      const Group& g = ds->groups[gi];
+      GroupInfo info;
+      if (g.number_of_terms > 0) registry_get(g.d_constraints, &info);
+      const int n = g.number_of_terms > 0 ? info.n : 0;
+      const int n_padded = (n + 255) / 256 * 256;
+      const int ksplit = static_cast<int>(
+          std::min<size_t>(ksplit_cap, choose_ksplit(n_padded / 256, freq_tiles)));
+      vp.basis = g.basis_function_id;
+      const float radius = info.has_radii ? info.radius : 1.0f;
+      if (!hip_ok(launch_prep_packets(g.d_constraints, g.d_radii, n, n_padded, c->step_size.x, c->step_size.y, u, v,
+                                      cull_factor(c, g.basis_function_id), packets, p->stream),
+                  "prep_packets launch") ||
+          !hip_ok(launch_direct_sum(packets, n_padded, ksplit, info.uniform_radius ? 1 : 0, radius, vp, partial,
+                                    p->stream),
+                  "direct_sum launch") ||
+          !hip_ok(launch_reduce_partials(partial, ksplit, nfreq, gi == 0 ? 1 : 0, acc, p->stream),
+                  "reduce_partials launch"))
+        return false;
+    }
+    if (!hip_ok(launch_finalize_spectrum(acc, nfreq, c->_scale, c->_d_freq_image, p->stream), "finalize launch"))
+      return false;
+  }
+  return hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
+                           c->_number_of_samples.y, p->tw_x, p->tw_y, c->_d_freq_image_arranged, c->_d_image,
+                           p->stream),
+                "c2r launch");
+}
+
 }  // namespace mv
 
 using namespace mv;
@@ -898,6 +1011,29 @@ void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
   hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
                     c->_number_of_samples.y, p->tw_x, p->tw_y, c->_d_freq_image_arranged, c->_d_image, p->stream),
          "c2r launch");
+}
+
+void vis_fourier_volume_rendering_noalloc(MeshlessDataset* ds, VisConfig* c) {
+  clear_error();
+  if (!ds || !c) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_noalloc: null argument");
+    return;
+  }
+  std::string why;
+  if (!config_valid(c, &why)) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+    return;
+  }
+  if (c->form != VIS_FORM_PLAIN) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "vis_fourier_volume_rendering_noalloc: renders the plain form only");
+    return;
+  }
+  if (!c->_d_image) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_noalloc: no _d_image (manual image not set)");
+    return;
+  }
+  vis_config_compute_scale(c);
+  render_noalloc(ds, c);
 }
 
 void vis_opengl_fourier_volume_rendering(MeshlessDataset*, VisConfig*, unsigned int) {
