@@ -152,9 +152,28 @@ struct RenderOut {
   int pad;
 };
 hipError_t launch_batch_finish(const RenderOut* r, const JobPart* jp, int n_renders, int cx, int cy, hipStream_t s);
+// ------------------------------------------------------------ C2R (fft.hip) ---
+// One axis of the 2-D C2R: the inverse DFT of length N as Stockham stages over LDS lines of
+// length L (L = N, or the power-of-two Bluestein length when N has a prime factor above 13).
+constexpr int kFftMaxStages = 16;
+struct FftAxis {
+  int N, L, nst, pad;
+  int radix[kFftMaxStages];
+  const float2* tw;     // W^j, then W^(128 j), j < 128 (W = exp(+2 pi i / L))
+  const float2* chirp;  // Bluestein: exp(+pi i n^2 / N), n < N; nullptr for a direct transform
+  const float2* gb;     // Bluestein: transform of the wrapped conj(chirp), divided by L
+};
+struct FftPlan {
+  FftAxis x{}, y{};
+  DevBuf mem;  // the tables
+};
+// Whether a line length is supported (1..16384; at most 8192 with a prime factor above 13).
+bool fft_length_supported(int n, std::string* why);
+// Tables of an Nx x Ny C2R (synchronous upload); false (error recorded) on failure.
+bool fft_plan_build(int nx, int ny, FftPlan* plan);
 // Batched C2R of n_renders spectra of one grid (cx, cy, Nx, Ny).
-hipError_t launch_c2r_batch(const RenderOut* r, int n_renders, int cx, int cy, int Nx, int Ny, const float2* tw_x,
-                            const float2* tw_y, hipStream_t s);
+hipError_t launch_c2r_batch(const RenderOut* r, int n_renders, int cx, int cy, int Nx, int Ny, const FftPlan& plan,
+                            hipStream_t s);
 
 // direct_sum.hip: plain FP32 direct sum over [first, first+count) packets.
 //   phi_kind: 0 = uniform radius (Phi hoisted per frequency), 1 = per-particle radius.
@@ -167,12 +186,12 @@ hipError_t launch_reduce_partials(const double2* partial, int ksplit, int nfreq,
 // F[i] = (scale*re, -scale*im) as float2 (slot 0 of _d_freq_image).
 hipError_t launch_finalize_spectrum(const double2* acc, int nfreq, float scale, float2* F, hipStream_t s);
 
-// fft.hip: 2-D C2R of the half-plane spectrum F[cy][2cx] into image[Nx][Ny];
-// work is the Nx*(Ny/2+1) complex buffer (_d_freq_image_arranged).
 // Zero `words` 32-bit words at p with a kernel (no memset node when a caller captures the stream).
 hipError_t launch_zero_words(void* p, size_t words, hipStream_t s);
 
-hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const float2* tw_x, const float2* tw_y,
-                      float2* work, float* image, hipStream_t s);
+// fft.hip: 2-D C2R of the half-plane spectrum F[cy][2cx] into image[Nx][Ny]; work is the
+// Nx*(Ny/2+1) complex buffer (_d_freq_image_arranged), used as the intermediate [cy][Nx].
+hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const FftPlan& plan, float2* work,
+                      float* image, hipStream_t s);
 
 }  // namespace mv

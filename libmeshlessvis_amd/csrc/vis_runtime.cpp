@@ -134,8 +134,7 @@ struct VisPlan {
   int device = 0;
   hipStream_t stream = nullptr;
   int nx = 0, ny = 0;
-  float2* tw_x = nullptr;
-  float2* tw_y = nullptr;
+  mv::FftPlan fft;  // tables of the C2R of the config's number_of_samples
   mv::DevBuf acc, partial, packets;
   mv::MfmaWorkspace mfma;
   BatchGraph batch;
@@ -147,29 +146,20 @@ struct VisPlan {
 
 namespace mv {
 
-static bool make_twiddles(int n, float2** out) {
-  if (*out) (void)hipFree(*out);
-  *out = nullptr;
-  if (n <= 0) return true;
-  std::vector<float2> h(n);
-  for (int k = 0; k < n; ++k) {
-    const double a = 2.0 * M_PI * static_cast<double>(k) / static_cast<double>(n);
-    h[k] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
-  }
-  if (!hip_ok(hipMalloc(reinterpret_cast<void**>(out), sizeof(float2) * n), "hipMalloc(twiddles)")) return false;
-  return hip_ok(hipMemcpy(*out, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice), "hipMemcpy(twiddles)");
-}
-
+// The C2R tables of a size this build supports; for another size the config is still created
+// (as the reference's vis_config_create, which validates nothing) and vis_config_check says why
+// it cannot render.
 static bool plan_set_samples(VisPlan* p, int nx, int ny) {
   p->nx = nx;
   p->ny = ny;
-  return make_twiddles(nx, &p->tw_x) && make_twiddles(ny, &p->tw_y);
+  p->fft.x.N = p->fft.y.N = 0;
+  if (!fft_length_supported(nx, nullptr) || !fft_length_supported(ny, nullptr)) return true;
+  return fft_plan_build(nx, ny, &p->fft);
 }
 
 static void plan_destroy(VisPlan* p) {
   if (!p) return;
-  if (p->tw_x) (void)hipFree(p->tw_x);
-  if (p->tw_y) (void)hipFree(p->tw_y);
+  p->fft.mem.release();
   p->acc.release();
   p->partial.release();
   p->packets.release();
@@ -256,7 +246,12 @@ static bool config_valid(const VisConfig* c, std::string* why) {
   if (c->_number_of_partial_sums < 1) { *why = "number_of_partial_sums < 1"; return false; }
   // Limits of this build (not in the reference).
   if (c->_cutoff_frequency.x < 0 || c->_cutoff_frequency.y < 0) { *why = "negative cutoff"; return false; }
-  if (c->_number_of_samples.x > 4096 || c->_number_of_samples.y > 4096) { *why = "number_of_samples above 4096"; return false; }
+  if (!fft_length_supported(c->_number_of_samples.x, why) || !fft_length_supported(c->_number_of_samples.y, why))
+    return false;
+  if (c->_plan->fft.x.N != c->_number_of_samples.x || c->_plan->fft.y.N != c->_number_of_samples.y) {
+    *why = "number_of_samples changed without vis_config_change_number_of_samples";
+    return false;
+  }
   if (c->form < VIS_FORM_PLAIN || c->form > VIS_FORM_MFMA_SPLIT_F16) { *why = "unknown form"; return false; }
   if (c->phi_mode != VIS_PHI_REFERENCE && c->phi_mode != VIS_PHI_ANALYTIC) { *why = "unknown phi_mode"; return false; }
   if (!(c->h_bin_ratio == 0.0f || c->h_bin_ratio > 1.0f)) { *why = "h_bin_ratio must be 0 (auto) or above 1"; return false; }
@@ -454,8 +449,7 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
                   static_cast<int>(c->cull_fully_aliased_terms)})
       key_put(&k, v);
     for (const void* q : {static_cast<const void*>(c->_d_freq_image), static_cast<const void*>(c->_d_image),
-                          static_cast<const void*>(c->_d_freq_image_arranged), static_cast<const void*>(c->_plan->tw_x),
-                          static_cast<const void*>(c->_plan->tw_y), static_cast<const void*>(c->_plan->acc.ptr),
+                          static_cast<const void*>(c->_d_freq_image_arranged), static_cast<const void*>(c->_plan->fft.mem.ptr), static_cast<const void*>(c->_plan->acc.ptr),
                           static_cast<const void*>(c->_plan->stream)})
       key_put(&k, q);
     key_put(&k, ds[i]->number_of_groups);
@@ -591,7 +585,7 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
          hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, q),
                 "finish launch") &&
          hip_ok(launch_c2r_batch(outs, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, c0->_number_of_samples.x,
-                                 c0->_number_of_samples.y, c0->_plan->tw_x, c0->_plan->tw_y, q),
+                                 c0->_number_of_samples.y, c0->_plan->fft, q),
                 "c2r launch");
   } else {
     ok = mfma_enqueue(specs, bg->ws.batch, q, nullptr);
@@ -601,7 +595,7 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
       ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, q),
                   "finalize launch") &&
            hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                             c->_number_of_samples.y, c->_plan->tw_x, c->_plan->tw_y, c->_d_freq_image_arranged,
+                             c->_number_of_samples.y, c->_plan->fft, c->_d_freq_image_arranged,
                              c->_d_image, q),
                   "c2r launch");
     }
@@ -757,7 +751,7 @@ static bool render_noalloc(MeshlessDataset* ds, VisConfig* c) {
       return false;
   }
   return hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                           c->_number_of_samples.y, p->tw_x, p->tw_y, c->_d_freq_image_arranged, c->_d_image,
+                           c->_number_of_samples.y, p->fft, c->_d_freq_image_arranged, c->_d_image,
                            p->stream),
                 "c2r launch");
 }
@@ -1009,7 +1003,7 @@ void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
     if (!spectrum(ds, c)) return;
   }
   hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                    c->_number_of_samples.y, p->tw_x, p->tw_y, c->_d_freq_image_arranged, c->_d_image, p->stream),
+                    c->_number_of_samples.y, p->fft, c->_d_freq_image_arranged, c->_d_image, p->stream),
          "c2r launch");
 }
 

@@ -127,18 +127,56 @@ def test_dataset_without_groups_renders_zero(form):
     assert not np.any(F) and not np.any(img)
 
 
-@pytest.mark.parametrize("samples", [(96, 80), (64, 256), (1024, 1024)])
+def fft_cutoff(samples, limit=(512, 512), block_length=64):
+    """The largest cutoff within `limit` and half the image with 2 cx cy a multiple of the block
+    length (the reference's vis_config_check rules, meshless_vis.cu:191-210)."""
+    best = (0, 0)
+    for cx in range(min(limit[0], samples[0] // 2), -1, -1):
+        for cy in range(min(limit[1], samples[1] // 2), -1, -1):
+            if (2 * cx * cy) % block_length == 0 and cx * cy > best[0] * best[1]:
+                best = (cx, cy)
+    return best
+
+
+# Powers of two, smooth mixed-radix lengths (2, 3, 5, 7, 11, 13), lengths with a large prime
+# factor (Bluestein: 17, 97, 131, 1018 = 2 * 509, 8191), and the longest lines (16384).
+FFT_SIZES = [(96, 80), (64, 256), (1024, 1024), (600, 600), (768, 512), (8192, 64), (64, 8192), (17, 9),
+             (25, 33), (97, 131), (1000, 1018), (8191, 66), (16384, 64), (64, 16384), (1001, 715)]
+
+
+@pytest.mark.parametrize("samples", FFT_SIZES)
 def test_fft_sizes(samples):
     rng = np.random.default_rng(3)
     groups = _small_groups(rng, 512, "GAUSSIAN", "var")
-    cutoff = (min(32, samples[0] // 2), min(40, samples[1] // 2))
+    cutoff = fft_cutoff(samples)
+    print(f"[case] fft {samples} cutoff {cutoff}", flush=True)
     F, img = gpu_render(groups, cutoff, samples, (0.8, 0.8), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0), "analytic",
                         block_length=64)
+    assert np.abs(F).max() > 0
     A = oracle.arrange64(F.astype(np.complex128), *cutoff, *samples)
     np_img = np.fft.irfft2(A, s=samples, norm="forward")
     e = oracle.rel_l2(img, np_img)
     print(f"fft {samples}: {e:.3e}")
     assert e <= 3e-6
+
+
+@pytest.mark.parametrize("samples", [(16385, 64), (64, 20000), (8209, 64), (16383, 64)])
+def test_fft_sizes_beyond_the_build_are_refused_by_check(samples):
+    """Lines above 16384 values, and above 8192 with a prime factor above 13 (8209 is prime,
+    16383 = 3 * 43 * 127), are outside this build: the config is created (as the reference's
+    vis_config_create, which validates nothing), vis_config_check says no, and a render reports
+    VIS_ERROR_INVALID_CONFIG."""
+    from libmeshlessvis_amd.abi import VisError
+
+    rng = np.random.default_rng(4)
+    ds = host.Dataset(_small_groups(rng, 64, "GAUSSIAN", "var"))
+    with host.Config(cutoff=(16, 16), samples=samples) as cfg:
+        assert not cfg.check()
+        cfg.register(ds)
+        with pytest.raises(VisError) as e:
+            cfg.render(ds)
+        assert e.value.name == "VIS_ERROR_INVALID_CONFIG"
+        cfg.unregister(ds)
 
 
 def test_errors_are_reported_not_fatal():
