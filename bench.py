@@ -34,6 +34,8 @@ PEAKS = {  # MI355X published dense peaks
 FLOPS_PER_TERM = 8.0  # one complex multiply-accumulate of w*Phi by exp(-i theta)
 # Arithmetic type of each form's products (the MFMA forms accumulate in FP32; the split forms
 # multiply hi/lo halves of that type; every form reduces its partial sums in FP64).
+TILES = {"default": 0, "half_x": 1, "half_y": 2, "quarter": 3}  # enum VisTile
+TILE_NAMES = {v: k for k, v in TILES.items()}
 DTYPE = {"plain": "f32", "mfma_f16": "f16", "mfma_split_f16": "f16", "mfma_split_bf16": "bf16"}
 
 
@@ -206,6 +208,8 @@ def batched_bench(args) -> None:
     if args.h_bin_ratio > 0.0:
         for c in cfgs:
             c.cs.h_bin_ratio = args.h_bin_ratio
+    for c in cfgs:
+        c.cs.tile = TILES[args.tile]
     for ds in sets:
         cfgs[0].register(ds)
     tuned = None
@@ -218,11 +222,13 @@ def batched_bench(args) -> None:
         tuned_ms = probe.autotune(sets[big], args.autotune)
         cs = probe.cs
         tuned = {"tolerance": args.autotune, "nodes": cs.number_of_interpolation_nodes,
-                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "spectrum_ms": tuned_ms, "timestep": big}
+                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "tile": TILE_NAMES[cs.tile], "spectrum_ms": tuned_ms,
+                 "timestep": big}
         for c in cfgs:
             c.cs.number_of_h_bins = 0
             c.cs.number_of_interpolation_nodes = cs.number_of_interpolation_nodes
             c.cs.h_bin_ratio = cs.h_bin_ratio
+            c.cs.tile = cs.tile
         log(f"autotune: {tuned} ({time.perf_counter() - t:.1f} s)")
     n_total = sum(wl.n for wl in wls)
     terms = sum(wl.terms_per_view * len(wl.views) for wl in wls)
@@ -371,6 +377,8 @@ def main() -> None:
                     help="before timing, let vis_autotune pick nodes/bin ratio within TOL of the plain kernel")
     ap.add_argument("--h-bin-ratio", type=float, default=0.0, metavar="R",
                     help="ratio of the h-bins of the MFMA forms (0: the library's default for the kind)")
+    ap.add_argument("--tile", default="default", choices=list(TILES),
+                    help="MFMA output tile variant (VisTile)")
     ap.add_argument("--cull", action="store_true",
                     help="render with cull_fully_aliased_terms set (drop terms wholly outside the period window)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -405,6 +413,8 @@ def main() -> None:
     if args.h_bin_ratio > 0.0:
         for c in cfgs:
             c.cs.h_bin_ratio = args.h_bin_ratio
+    for c in cfgs:
+        c.cs.tile = TILES[args.tile]
     cfgs[0].register(ds)
     tuned = None
     if args.autotune > 0 and form != "plain":
@@ -412,11 +422,12 @@ def main() -> None:
         tuned_ms = cfgs[0].autotune(ds, args.autotune)
         cs = cfgs[0].cs
         tuned = {"tolerance": args.autotune, "nodes": cs.number_of_interpolation_nodes,
-                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "spectrum_ms": tuned_ms}
+                 "h_bin_ratio": round(cs.h_bin_ratio, 4), "tile": TILE_NAMES[cs.tile], "spectrum_ms": tuned_ms}
         for c in cfgs[1:]:
             c.cs.number_of_h_bins = 0
             c.cs.number_of_interpolation_nodes = cs.number_of_interpolation_nodes
             c.cs.h_bin_ratio = cs.h_bin_ratio
+            c.cs.tile = cs.tile
         log(f"autotune: {tuned} ({time.perf_counter() - t:.1f} s)")
     log(f"registered; timing form={form}")
     elapsed, launches, kernel_ms = time_renders(cfgs, ds, args.steps, args.warmup)
@@ -533,6 +544,7 @@ def main() -> None:
             "form": form,
             "autotune": tuned,
             "h_bin_ratio": args.h_bin_ratio or None,
+            "tile": args.tile,
             "cull_fully_aliased_terms": bool(args.cull),
         },
         "views_per_s": views / elapsed,

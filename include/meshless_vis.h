@@ -69,6 +69,18 @@ enum VisError
 	VIS_ERROR_NOT_REGISTERED = 6
 };
 
+/* Additive: output tile of the MFMA direct-sum kernels (|x| rows x y columns per workgroup),
+   relative to each kernel's default: 128 x 128 for the Gaussian / uniform-radius kernel, 64 x 64
+   for the two-node SPH/Wendland kernel.  The split forms build every variant; the single-FP16
+   form and the kernel for 3 or more interpolation nodes (128 x 64) keep their default. */
+enum VisTile
+{
+	VIS_TILE_DEFAULT = 0,
+	VIS_TILE_HALF_X = 1,   /* half the |x| rows:   64 x 128, 32 x 64 */
+	VIS_TILE_HALF_Y = 2,   /* half the y columns:  128 x 64, 64 x 32 */
+	VIS_TILE_QUARTER = 3   /* both halved:         64 x 64,  32 x 32 */
+};
+
 typedef struct VisPlan VisPlan; /* opaque, owned by the library */
 
 /* Replaces include/meshless_vis.h:39-79. */
@@ -110,6 +122,7 @@ typedef struct
 	int number_of_interpolation_nodes;  /* MFMA form, SPH/Wendland radii: nodes per bin, 0 = auto */
 	float h_bin_ratio;                  /* MFMA form: ratio h_hi/h_lo of the radius bins, 0 = auto
 	                                       (used when number_of_h_bins is 0)                         */
+	int tile;                           /* MFMA form: enum VisTile, default VIS_TILE_DEFAULT      */
 } VisConfig;
 
 /* meshless_vis.h:81 / meshless_vis.cu:129-176.  Returns 0 (and sets the last error) on failure. */
@@ -184,13 +197,16 @@ void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, V
    streams or threads) race. */
 void vis_fourier_volume_rendering_noalloc(MeshlessDataset* meshless_dataset, VisConfig* vis_config);
 
-/* Picks number_of_interpolation_nodes and h_bin_ratio for the MFMA form on this
-   (registered) dataset: every candidate is timed on the config's stream and its
-   spectrum compared with the plain FP32 kernel's; the fastest candidate whose
+/* Picks number_of_interpolation_nodes, h_bin_ratio and tile for the MFMA form on
+   this (registered) dataset: every candidate is timed on the config's stream and
+   its spectrum compared with the plain FP32 kernel's; the fastest candidate whose
    relative L2 difference is at most `tolerance` is kept (the most accurate one
-   when none is), with number_of_h_bins set to 0 (bins from the ratio).  Leaves
-   the config's other fields as they were and overwrites _d_freq_image.  Returns
-   the kept candidate's spectrum time in milliseconds, or a negative value on error. */
+   when none is), with number_of_h_bins set to 0 (bins from the ratio).  The
+   (nodes, ratio) candidates are timed first with the config's tile, then the
+   kept setting with the other VisTile variants of the kernel it runs (split
+   forms).  Leaves the config's other fields as they were and overwrites
+   _d_freq_image.  Returns the kept candidate's spectrum time in milliseconds, or
+   a negative value on error. */
 double vis_autotune(MeshlessDataset* meshless_dataset, VisConfig* vis_config, double tolerance);
 
 /* Kernel timing with HIP events on the config's stream around the direct-sum

@@ -22,6 +22,7 @@ def library_path() -> Path:
 BASIS = {"SPH": 0, "GAUSSIAN": 1, "WENDLAND_D3_C2": 2}
 FORM = {"plain": 0, "mfma_f16": 1, "mfma_split_bf16": 2, "mfma_split_f16": 3}
 PHI_MODE = {"reference": 0, "analytic": 1}
+TILE = {"default": 0, "half_x": 1, "half_y": 2, "quarter": 3}
 ERRORS = {
     0: "VIS_SUCCESS",
     1: "VIS_ERROR_INVALID_VALUE",
@@ -87,6 +88,7 @@ class VisConfig(C.Structure):
         ("number_of_h_bins", C.c_int),
         ("number_of_interpolation_nodes", C.c_int),
         ("h_bin_ratio", C.c_float),
+        ("tile", C.c_int),
     ]
 
 

@@ -32,7 +32,7 @@ struct MfmaPlanned;
 // Work tables of a prepared batch (device pointers into the workspace).
 struct MfmaBatch {
   std::vector<MfmaPlanned> plans;
-  std::vector<std::tuple<int, int, int>> keys;  // (kind, basis, nodes) of every launch
+  std::vector<std::tuple<int, int, int, int>> keys;  // (kind, basis, nodes, tile) of every launch
   std::vector<int> job_begin, item_begin;
   const MfmaJob* jobs = nullptr;
   const MfmaItem* items = nullptr;

@@ -49,15 +49,20 @@ namespace mv {
 constexpr int kStage = 32;
 constexpr int kRowPitch = kStage / 2 + 4;
 
-template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
-__global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
-                                                                    const MfmaItem* __restrict__ items, MfmaShared sh,
-                                                                    int* __restrict__ counter) {
+// Threads of the sum kernel with a TX x TY tile: 4 producer waves and one consumer wave per
+// 32 |x| x 64 y block.
+constexpr int sum_threads(int TX, int TY) { return (kProducerWaves + (TX / 32) * (TY / 64)) * 64; }
+
+template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE, int TX, int TY>
+__global__ __launch_bounds__(sum_threads(TX, TY), 1) void mfma_sum_kernel(const MfmaJob* __restrict__ jobs,
+                                                                            const MfmaItem* __restrict__ items,
+                                                                            MfmaShared sh, int* __restrict__ counter) {
+  static_assert(TX % 32 == 0 && TY % 64 == 0 && TX <= kTileX && TY <= kTileY, "tile");
   // [buffer][re_hi, re_lo, im_hi, im_lo][row * kRowPitch + particle pair]; A rows are |x|, B rows are y.
-  // Together exactly 160 KiB: the work-item index lives in a padding word (words 16..19 of
-  // every row are never read or written by the operand code).
-  __shared__ uint32_t ldsA[2][4][kTileX * kRowPitch];
-  __shared__ uint32_t ldsB[2][4][kTileY * kRowPitch];
+  // With the default 128 x 128 tile together exactly 160 KiB: the work-item index lives in a
+  // padding word (words 16..19 of every row are never read or written by the operand code).
+  __shared__ uint32_t ldsA[2][4][TX * kRowPitch];
+  __shared__ uint32_t ldsB[2][4][TY * kRowPitch];
   int* next_item = reinterpret_cast<int*>(&ldsA[0][0][kStage / 2]);
 
   const int tid = threadIdx.x;
@@ -77,8 +82,8 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
     const MfmaItem item = items[it];
     const MfmaJob job = jobs[item.job];
     const int tile_x = item.tile % job.tiles_x;
-    const int xa0 = 1 + tile_x * kTileX;  // first |x| of the tile
-    const int y0 = (item.tile / job.tiles_x) * kTileY;
+    const int xa0 = 1 + tile_x * TX;  // first |x| of the tile
+    const int y0 = (item.tile / job.tiles_x) * TY;
     const float4 bi = job.bininfo ? job.bininfo[item.bin] : job.bin0;
     const float hh_ref = sh.ref_hi ? bi.y : bi.x;
     // Particles [p0, p1) of the item, in stages of 32 (the last one may be partly empty:
@@ -89,27 +94,32 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
       const int pp = lane & 15;  // particles 2pp, 2pp+1 of the stage
-      // 0..15: A rows and B rows gg + 16m (m < 8).  A wave's four row groups are 4 apart, so
+      // 0..15: A rows gg + 16m (m < TX / 16) and B rows gg + 16m (m < TY / 16).  A wave's four
+      // row groups are 4 apart, so
       // its stores (rows 20 words apart) hit each of the 32 LDS banks exactly twice; row
       // groups 1 apart would collide 3-way.
       const int gg = (lane >> 4) * 4 + wave;
       // Row gg + 16m: phase = P0 + m * 16 * (phase step), exact in 32-bit modular arithmetic.
       const uint32_t xa_first = static_cast<uint32_t>(xa0 + gg);
       const uint32_t y_first = static_cast<uint32_t>(y0 + gg);
-      float ca[8], cb[8];
+      constexpr int MA = TX / 16, MB = TY / 16;
+      float ca[MA], cb[MB];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) {
+      for (int m = 0; m < MA; ++m) {
         const float fu = sh.sx * static_cast<float>(static_cast<int>(xa_first) + 16 * m);
-        const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
         ca[m] = sh.sigma_pi_log2e * fu * fu;
+      }
+#pragma unroll
+      for (int m = 0; m < MB; ++m) {
+        const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
         cb[m] = sh.sigma_pi_log2e * fv * fv;
       }
       // x = 0: A(0, i) = 1, so S(0, y) = E(0, y) * sum_i B(i, y), summed here in FP32 by the
       // tiles of the first |x| range (cx = 0: an empty half plane).
       const bool zero_col = tile_x == 0 && sh.cx > 0;
-      float zr[8], zi[8];
+      float zr[MB], zi[MB];
 #pragma unroll
-      for (int m = 0; m < 8; ++m) zr[m] = zi[m] = 0.0f;
+      for (int m = 0; m < MB; ++m) zr[m] = zi[m] = 0.0f;
       // Loads without a branch, clamped into the item: a conditional load would make the
       // compiler wait for it at the branch's join, right after issuing it.  Slots past p1
       // are zeroed through their weight in generate.
@@ -136,7 +146,7 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
           dB[u] = q[u].pv << 4;
         }
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {  // A(|x|, i)
+        for (int m = 0; m < MA; ++m) {  // A(|x|, i)
           float re[2], im[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -150,10 +160,10 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
               im[u] *= amp;
             }
           }
-          store_pair<OP, SPLIT, kTileX * kRowPitch>(ldsA[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
+          store_pair<OP, SPLIT, TX * kRowPitch>(ldsA[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
         }
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {  // B(i, y)
+        for (int m = 0; m < MB; ++m) {  // B(i, y)
           float re[2], im[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -168,7 +178,7 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
             zr[m] += re[0] + re[1];
             zi[m] += im[0] + im[1];
           }
-          store_pair<OP, SPLIT, kTileY * kRowPitch>(ldsB[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
+          store_pair<OP, SPLIT, TY * kRowPitch>(ldsB[buf], (gg + 16 * m) * kRowPitch + pp, re, im);
         }
       };
       // Packets are loaded one stage ahead of their use, so the load's latency
@@ -197,7 +207,7 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
       if (zero_col) {
         // Sum over the 16 lanes of a row group (pp = 0..15), then one lane per row writes.
 #pragma unroll
-        for (int m = 0; m < 8; ++m) {
+        for (int m = 0; m < MB; ++m) {
 #pragma unroll
           for (int d = 1; d < 16; d <<= 1) {
             zr[m] += __shfl_xor(zr[m], d);
@@ -220,9 +230,9 @@ __global__ __launch_bounds__(kSumThreads, 1) void mfma_sum_kernel(const MfmaJob*
     }
 
     // ---- consumer: 32 |x| rows x 64 y columns; real products rr, ii, ri, ir ----
-    const int c = wave - kProducerWaves;  // 0..7
-    const int wx = c & 3;
-    const int wy = c >> 2;
+    const int c = wave - kProducerWaves;  // 0 .. (TX / 32) (TY / 64) - 1
+    const int wx = c % (TX / 32);
+    const int wy = c / (TX / 32);
     f32x16_t acc[4][2];  // [Ar.Br, Ai.Bi, Ar.Bi, Ai.Br][column block]
 #pragma unroll
     for (int q = 0; q < 4; ++q) acc[q][0] = acc[q][1] = (f32x16_t){};
@@ -337,11 +347,31 @@ hipError_t launch_zero_words(void* p, size_t words, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------ host ---
+template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE, int TX, int TY>
+static void launch_sum_tile(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items,
+                            const MfmaShared& sh, int* counter) {
+  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE, TX, TY>), dim3(grid),
+                     dim3(sum_threads(TX, TY)), 0, s, jobs, items, sh, counter);
+}
+
+// The tile variants (sum_tile) are built for the split forms; the single-FP16 form keeps the
+// default tile.
 template <int OP, bool SPLIT, bool GAUSS, int BASIS, int MODE>
 static void launch_sum(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
                        int* counter) {
-  hipLaunchKernelGGL((mfma_sum_kernel<OP, SPLIT, GAUSS, BASIS, MODE>), dim3(grid), dim3(kSumThreads), 0, s, jobs, items,
-                     sh, counter);
+  switch (SPLIT ? sh.tile : VIS_TILE_DEFAULT) {
+    case VIS_TILE_HALF_X:
+      launch_sum_tile<OP, SPLIT, GAUSS, BASIS, MODE, kTileX / 2, kTileY>(grid, s, jobs, items, sh, counter);
+      break;
+    case VIS_TILE_HALF_Y:
+      launch_sum_tile<OP, SPLIT, GAUSS, BASIS, MODE, kTileX, kTileY / 2>(grid, s, jobs, items, sh, counter);
+      break;
+    case VIS_TILE_QUARTER:
+      launch_sum_tile<OP, SPLIT, GAUSS, BASIS, MODE, kTileX / 2, kTileY / 2>(grid, s, jobs, items, sh, counter);
+      break;
+    default:
+      launch_sum_tile<OP, SPLIT, GAUSS, BASIS, MODE, kTileX, kTileY>(grid, s, jobs, items, sh, counter);
+  }
 }
 
 template <int OP, bool SPLIT>
@@ -465,10 +495,17 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   // first range); binned kernel tiles (T >= 3) cover j = x + cx - 1 in [0, 2cx).
   // Two-node binned kernel: |x| rows as the sum kernel.
   const bool paired = binned && out->T == 2;
-  job.tiles_x = !binned ? std::max(1, (spec.vp.cx + kTileX - 1) / kTileX)
-                : paired ? std::max(1, (spec.vp.cx + kPairTileX - 1) / kPairTileX)
-                         : (2 * spec.vp.cx + kBinnedTileX - 1) / kBinnedTileX;
-  const int ty = !binned ? kTileY : paired ? kPairTileY : kBinnedTileY;
+  // The tile variant: split forms only (the single-FP16 form and the T >= 3 kernel keep theirs).
+  const bool split = c->form != VIS_FORM_MFMA_F16;
+  out->tile = split && (!binned || paired) ? c->tile : VIS_TILE_DEFAULT;
+  int tx, ty;
+  if (!binned)
+    tile_dims(out->tile, kTileX, kTileY, &tx, &ty);
+  else if (paired)
+    tile_dims(out->tile, kPairTileX, kPairTileY, &tx, &ty);
+  else
+    tx = kBinnedTileX, ty = kBinnedTileY;
+  job.tiles_x = !binned || paired ? std::max(1, (spec.vp.cx + tx - 1) / tx) : (2 * spec.vp.cx + tx - 1) / tx;
   const int tiles = job.tiles_x * ((spec.vp.cy + ty - 1) / ty);
   const long long total_steps = bin_step.back();
   const int cap = binned ? 4096 : kMaxChunkSteps;
@@ -560,7 +597,8 @@ static std::vector<unsigned long long> plan_key(const std::vector<MfmaJobSpec>& 
           static_cast<unsigned long long>(sp.n), static_cast<unsigned long long>(sp.vp.cx),
           static_cast<unsigned long long>(sp.vp.cy), static_cast<unsigned long long>(sp.vp.basis),
           static_cast<unsigned long long>(sp.c->number_of_h_bins),
-          static_cast<unsigned long long>(sp.c->number_of_interpolation_nodes), static_cast<unsigned long long>(ratio)})
+          static_cast<unsigned long long>(sp.c->number_of_interpolation_nodes), static_cast<unsigned long long>(ratio),
+          static_cast<unsigned long long>(sp.c->tile), static_cast<unsigned long long>(sp.c->form)})
       k.push_back(v);
   }
   return k;
@@ -584,7 +622,7 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
            a.vp.cx == b.vp.cx && a.vp.cy == b.vp.cy && a.vp.basis == b.vp.basis &&
            a.c->number_of_h_bins == b.c->number_of_h_bins &&
            a.c->number_of_interpolation_nodes == b.c->number_of_interpolation_nodes &&
-           a.c->h_bin_ratio == b.c->h_bin_ratio;
+           a.c->h_bin_ratio == b.c->h_bin_ratio && a.c->tile == b.c->tile && a.c->form == b.c->form;
   };
   std::vector<size_t> first(specs.size());
   for (size_t k = 0; k < specs.size(); ++k) {
@@ -623,9 +661,9 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
     pl.job.packets = ws->packets.as<Packet>() + pl.pk_off;
     pl.job.partial = ws->partial.as<float2>() + pl.part_off;
   }
-  // One persistent launch per kernel variant present (kind, basis, nodes).
+  // One persistent launch per kernel variant present (kind, basis, nodes, tile).
   auto key_of = [](const MfmaPlanned& pl) {
-    return std::make_tuple(pl.kind, pl.kind == KIND_GAUSS ? static_cast<int>(GAUSSIAN) : pl.basis, pl.T);
+    return std::make_tuple(pl.kind, pl.kind == KIND_GAUSS ? static_cast<int>(GAUSSIAN) : pl.basis, pl.T, pl.tile);
   };
   for (auto& pl : bt->plans)
     if (std::find(bt->keys.begin(), bt->keys.end(), key_of(pl)) == bt->keys.end()) bt->keys.push_back(key_of(pl));
@@ -715,6 +753,7 @@ bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt
   if (ev && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
   for (size_t k = 0; k < bt.keys.size(); ++k) {
     const int kind = std::get<0>(bt.keys[k]), basis = std::get<1>(bt.keys[k]), T = std::get<2>(bt.keys[k]);
+    sh.tile = std::get<3>(bt.keys[k]);
     sh.n_items = bt.item_begin[k + 1] - bt.item_begin[k];
     sh.sigma_pi_log2e = kind == KIND_GAUSS ? sigma * kPi * kLog2e : 0.0f;
     const int grid = std::max(1, std::min(sh.n_items, device_cus()));

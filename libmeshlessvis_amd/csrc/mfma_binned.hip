@@ -242,27 +242,29 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 // the MFMA's K index is the particle.  x = 0 (A = 1) is the column sum of B_t,
 // kept by the waves that generate B.
 //
-// Tile: 64 |x| rows x 64 columns, 12 waves.  Waves 0-3 generate the operands of
-// the next 32-particle stage (bins are padded to 32 particles, so a stage lies in
-// one bin); waves 4-11 multiply the current one: consumer c = wave - 4 holds node
-// t = c / 4 of the 32 x 32 block c % 4 - 4 products of accumulators and the folded
-// S(+-x) sums.  Waves w, w + 4 and w + 8 share a SIMD: one producer and both nodes'
-// consumers of block w.  An item owns chunk slots 2 * chunk and 2 * chunk + 1 (the
+// Tile: TX |x| rows x TY columns (default 64 x 64: 12 waves).  Waves 0-3 generate the
+// operands of the next 32-particle stage (bins are padded to 32 particles, so a stage lies
+// in one bin); the other waves multiply the current one: consumer c = wave - 4 holds node
+// t = c / NB of the 32 x 32 block c % NB (NB = TX TY / 1024 blocks) - 4 products of
+// accumulators and the folded S(+-x) sums.  With the default tile, waves w, w + 4 and w + 8
+// share a SIMD: one producer and both nodes' consumers of block w.  An item owns chunk slots 2 * chunk and 2 * chunk + 1 (the
 // reduction adds them): node t's consumers write its folded sums into slot
 // 2 * chunk + t and its producers the x = 0 column.
 constexpr int kPairStage = 2 * kStep;             // particles per stage
 constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
-constexpr int kPairThreads = 768;  // 4 producer waves, 8 consumer waves (block x node)
-constexpr int kPairAWords = 2 * 4 * kPairTileX * kPairPitch;      // [buf][re_hi, re_lo, im_hi, im_lo][row]
-constexpr int kPairBWords = 2 * 2 * 4 * kPairTileY * kPairPitch;  // [buf][node][...][row]
+// 4 producer waves and two consumer waves (nodes 0, 1) per 32 x 32 block of the tile.
+constexpr int pair_threads(int TX, int TY) { return (kProducerWaves + 2 * (TX / 32) * (TY / 32)) * 64; }
 
-template <int OP, bool SPLIT, int BASIS, int MODE>
-__global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const MfmaJob* __restrict__ jobs,
-                                                                             const MfmaItem* __restrict__ items,
-                                                                             MfmaShared sh, int* __restrict__ counter) {
+template <int OP, bool SPLIT, int BASIS, int MODE, int TX, int TY>
+__global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kernel(
+    const MfmaJob* __restrict__ jobs, const MfmaItem* __restrict__ items, MfmaShared sh, int* __restrict__ counter) {
+  static_assert(TX % 32 == 0 && TY % 32 == 0, "tile");
   constexpr int T = 2;
-  constexpr int kA = kPairTileX * kPairPitch;  // words of one A part
-  constexpr int kB = kPairTileY * kPairPitch;  // words of one B part
+  constexpr int NB = (TX / 32) * (TY / 32);  // 32 x 32 blocks of the tile
+  constexpr int kA = TX * kPairPitch;  // words of one A part
+  constexpr int kB = TY * kPairPitch;  // words of one B part
+  constexpr int kPairAWords = 2 * 4 * kA;      // [buf][re_hi, re_lo, im_hi, im_lo][row]
+  constexpr int kPairBWords = 2 * 2 * 4 * kB;  // [buf][node][...][row]
   __shared__ uint32_t lds[kPairAWords + kPairBWords];
   // The work-item index lives in a padding word (words 16..19 of a row are never read or
   // written by the operand code).
@@ -284,8 +286,8 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
     const MfmaItem item = items[it];
     const MfmaJob job = jobs[item.job];
     const int tile_x = item.tile % job.tiles_x;
-    const int xa0 = 1 + tile_x * kPairTileX;
-    const int y0 = (item.tile / job.tiles_x) * kPairTileY;
+    const int xa0 = 1 + tile_x * TX;
+    const int y0 = (item.tile / job.tiles_x) * TY;
     // Steps [s0, s1) in stages of two steps; s0 and s1 are even (bins and work items are).
     const int n_stages = (item.s1 - item.s0) / 2;
     auto stage_bin = [&](int k) -> int {
@@ -306,7 +308,8 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
     if (wave < kProducerWaves) {
       // ---- producer: operands of stage k+1 while the consumers multiply stage k ----
       const int pp = lane & 15;  // particles 2pp, 2pp+1 of the stage
-      // 0..15: A rows and B rows gg + 16m (m < 4).  A wave's four row groups are 4 apart, so
+      // 0..15: A rows gg + 16m (m < TX / 16) and B rows gg + 16m (m < TY / 16).  A wave's four
+      // row groups are 4 apart, so
       // its stores (rows 20 words apart) hit each of the 32 LDS banks exactly twice; row
       // groups 1 apart would collide 3-way.
       const int gg = (lane >> 4) * 4 + wave;
@@ -329,16 +332,17 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
       // their node-weighted sum over the item's bins (Zr, Zi).  Each column's 16 lanes
       // (pp = 0..15) are summed once, at the end of the item, by shuffles in a fixed order:
       // every column has one owner row group, so there is nothing to share through LDS.
-      float zr[T][4], zi[T][4], Zr[T][4], Zi[T][4];
+      constexpr int MA = TX / 16, MB = TY / 16;
+      float zr[T][MB], zi[T][MB], Zr[T][MB], Zi[T][MB];
 #pragma unroll
-      for (int m = 0; m < 4; ++m)
+      for (int m = 0; m < MB; ++m)
 #pragma unroll
         for (int t = 0; t < T; ++t) zr[t][m] = zi[t][m] = Zr[t][m] = Zi[t][m] = 0.0f;
       auto zfold = [&](int b) {
         const float* nd = job.nodes + b * kMaxNodes;
         const float ws = job.bininfo[b].z;
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
+        for (int m = 0; m < MB; ++m) {
           const float fv = sh.sy * static_cast<float>(static_cast<int>(y_first) + 16 * m);
 #pragma unroll
           for (int t = 0; t < T; ++t) {
@@ -359,7 +363,7 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
           dB[u] = a.q[u].pv << 4;
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {  // A(|x|, i)
+        for (int m = 0; m < MA; ++m) {  // A(|x|, i)
           float re[2], im[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
           wl[u][1] = a.q[u].w * a.l[u].y;
         }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {  // B_t(i, y)
+        for (int m = 0; m < MB; ++m) {  // B_t(i, y)
           float c0[2], s0[2];
 #pragma unroll
           for (int u = 0; u < 2; ++u) {
@@ -450,7 +454,7 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
 #pragma unroll
         for (int t = 0; t < T; ++t)
 #pragma unroll
-          for (int m = 0; m < 4; ++m) {
+          for (int m = 0; m < MB; ++m) {
 #pragma unroll
             for (int d = 1; d < 16; d <<= 1) {
               Zr[t][m] += __shfl_xor(Zr[t][m], d);
@@ -469,10 +473,10 @@ __global__ __launch_bounds__(kPairThreads, 1) void mfma_binned_pair_kernel(const
     // Waves w, w + 4 and w + 8 share SIMD w: the producer w and the two consumers of block w
     // (nodes 0 and 1), so every SIMD interleaves one wave's operand generation with two
     // waves' MFMAs and folds.
-    const int c = wave - kProducerWaves;  // 0..7
-    const int blk = c & 3, t = c >> 2;
-    const int wx = blk & 1;
-    const int wy = blk >> 1;
+    const int c = wave - kProducerWaves;  // 0 .. 2 NB - 1
+    const int blk = c % NB, t = c / NB;
+    const int wx = blk % (TX / 32);
+    const int wy = blk / (TX / 32);
     f32x16_t acc[4], sp[2], sm[2];  // [Prr, Pii, Pri, Pir] of node t; folded S(+x), S(-x)
     acc[0] = acc[1] = acc[2] = acc[3] = (f32x16_t){};
     sp[0] = sp[1] = sm[0] = sm[1] = (f32x16_t){};
@@ -774,8 +778,25 @@ static void launch_bt(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaIt
 template <int OP, bool SPLIT, int BASIS, int MODE>
 static void launch_pair(int grid, hipStream_t s, const MfmaJob* jobs, const MfmaItem* items, const MfmaShared& sh,
                         int* counter) {
-  hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE>), dim3(grid), dim3(kPairThreads), 0, s, jobs, items,
-                     sh, counter);
+  // Tile variants (tile_dims) for the split forms; the single-FP16 form keeps the default.
+  constexpr int X = kPairTileX, Y = kPairTileY;
+  switch (SPLIT ? sh.tile : VIS_TILE_DEFAULT) {
+    case VIS_TILE_HALF_X:
+      hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE, X / 2, Y>), dim3(grid),
+                         dim3(pair_threads(X / 2, Y)), 0, s, jobs, items, sh, counter);
+      break;
+    case VIS_TILE_HALF_Y:
+      hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE, X, Y / 2>), dim3(grid),
+                         dim3(pair_threads(X, Y / 2)), 0, s, jobs, items, sh, counter);
+      break;
+    case VIS_TILE_QUARTER:
+      hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE, X / 2, Y / 2>), dim3(grid),
+                         dim3(pair_threads(X / 2, Y / 2)), 0, s, jobs, items, sh, counter);
+      break;
+    default:
+      hipLaunchKernelGGL((mfma_binned_pair_kernel<OP, SPLIT, BASIS, MODE, X, Y>), dim3(grid), dim3(pair_threads(X, Y)),
+                         0, s, jobs, items, sh, counter);
+  }
 }
 
 template <int OP, bool SPLIT>

@@ -34,8 +34,17 @@ struct MfmaShared {
   float sigma_pi_log2e;  // Gaussian kind: +-pi*log2(e); 0 otherwise
   int ref_hi;            // Gaussian h_ref: 0 the bin's lower edge, 1 its upper edge
   int n_items;
-  int pad;
+  int tile;  // enum VisTile of the launch's jobs
 };
+
+// Tile (|x| rows, y columns per workgroup) of the sum kernel and of the two-node binned kernel
+// for a VisTile value: the default, half the |x| rows, half the y columns, or both halved.
+// The split forms build every variant; the single-FP16 form and the T >= 3 binned kernel keep
+// their default tile.
+inline void tile_dims(int tile, int dx, int dy, int* tx, int* ty) {
+  *tx = (tile == VIS_TILE_HALF_X || tile == VIS_TILE_QUARTER) ? dx / 2 : dx;
+  *ty = (tile == VIS_TILE_HALF_Y || tile == VIS_TILE_QUARTER) ? dy / 2 : dy;
+}
 
 
 // Two values rounded to nearest even into one word (v_cvt_pk_f16_f32 / v_cvt_pk_bf16_f32).

@@ -27,6 +27,7 @@ struct MfmaItem {
 
 struct MfmaPlanned {
   int kind = 0, basis = 0, T = 1;
+  int tile = 0;  // enum VisTile of the kernel that runs the job
   const BinCache* bc = nullptr;
   int n_slots = 0, nchunks = 0;
   long long tile_steps = 0;  // steps x tiles: the job's work

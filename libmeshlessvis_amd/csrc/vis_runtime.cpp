@@ -255,6 +255,7 @@ static bool config_valid(const VisConfig* c, std::string* why) {
   if (c->form < VIS_FORM_PLAIN || c->form > VIS_FORM_MFMA_SPLIT_F16) { *why = "unknown form"; return false; }
   if (c->phi_mode != VIS_PHI_REFERENCE && c->phi_mode != VIS_PHI_ANALYTIC) { *why = "unknown phi_mode"; return false; }
   if (!(c->h_bin_ratio == 0.0f || c->h_bin_ratio > 1.0f)) { *why = "h_bin_ratio must be 0 (auto) or above 1"; return false; }
+  if (c->tile < VIS_TILE_DEFAULT || c->tile > VIS_TILE_QUARTER) { *why = "unknown tile"; return false; }
   return true;
 }
 
@@ -411,7 +412,7 @@ static bool batch_fusable(MeshlessDataset** ds, VisConfig** cf, int n) {
         c->step_size.x != c0->step_size.x || c->step_size.y != c0->step_size.y ||
         c->number_of_h_bins != c0->number_of_h_bins ||
         c->number_of_interpolation_nodes != c0->number_of_interpolation_nodes ||
-        c->h_bin_ratio != c0->h_bin_ratio || ds[i]->number_of_groups < 1)
+        c->h_bin_ratio != c0->h_bin_ratio || c->tile != c0->tile || ds[i]->number_of_groups < 1)
       return false;
     // A group filled by the caller (no registry entry) is re-read at every render; a
     // captured graph could not follow it, so such a batch renders view by view.
@@ -445,7 +446,7 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
                     c->v_axis.z, c->h_bin_ratio})
       key_put(&k, f);
     for (int v : {c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y,
-                  c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes,
+                  c->form, c->phi_mode, c->number_of_h_bins, c->number_of_interpolation_nodes, c->tile,
                   static_cast<int>(c->cull_fully_aliased_terms)})
       key_put(&k, v);
     for (const void* q : {static_cast<const void*>(c->_d_freq_image), static_cast<const void*>(c->_d_image),
@@ -1145,10 +1146,11 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
                                  "hipMemcpy(spectrum)") &&
                           hip_ok(hipStreamSynchronize(p->stream), "synchronize"));
   };
-  const int form = c->form, nodes0 = c->number_of_interpolation_nodes, bins0 = c->number_of_h_bins;
+  const int form = c->form, nodes0 = c->number_of_interpolation_nodes, bins0 = c->number_of_h_bins,
+            tile0 = c->tile;
   const float ratio0 = c->h_bin_ratio;
   double best_ms = -1.0, best_err = 0.0;
-  int best_nodes = nodes0;
+  int best_nodes = nodes0, best_tile = tile0;
   float best_ratio = ratio0;
   bool best_ok = false;
   std::vector<float2> ref, got;
@@ -1158,20 +1160,14 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
   c->form = form;
   double ref_norm = 0.0;
   for (const float2& z : ref) ref_norm += static_cast<double>(z.x) * z.x + static_cast<double>(z.y) * z.y;
-  std::vector<std::pair<int, float>> cand;
-  if (binned)
-    cand = {{2, 1.005f}, {2, 1.01f}, {2, 1.02f}, {3, 1.03f}, {3, 1.04f}, {3, 1.06f}, {4, 1.1f}, {4, 1.15f},
-            {5, 1.2f}, {5, 1.3f}};
-  else if (gauss)
-    cand = {{nodes0, 1.25f}, {nodes0, 1.5f}, {nodes0, 2.0f}, {nodes0, 4.0f}};
-  else
-    cand = {{nodes0, ratio0}};
-  c->number_of_h_bins = 0;
-  for (size_t k = 0; ok && k < cand.size(); ++k) {
-    c->number_of_interpolation_nodes = cand[k].first;
-    c->h_bin_ratio = cand[k].second;
+  // Times one candidate and keeps it if it is the fastest within the tolerance (or, while none
+  // is, the most accurate).
+  auto consider = [&](int nodes, float ratio, int tile) {
+    c->number_of_interpolation_nodes = nodes;
+    c->h_bin_ratio = ratio;
+    c->tile = tile;
     double ms = 0.0;
-    if (!(ok = timed(&ms) && fetch(&got))) break;
+    if (!(ok = timed(&ms) && fetch(&got))) return;
     double d = 0.0;
     for (size_t i = 0; i < nfreq; ++i) {
       const double dx = static_cast<double>(got[i].x) - ref[i].x, dy = static_cast<double>(got[i].y) - ref[i].y;
@@ -1183,19 +1179,38 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
       best_ms = ms;
       best_err = err;
       best_ok = meets;
-      best_nodes = cand[k].first;
-      best_ratio = cand[k].second;
+      best_nodes = nodes;
+      best_ratio = ratio;
+      best_tile = tile;
     }
-  }
+  };
+  // First the interpolation nodes and bin ratio (with the config's tile), then the tile variants
+  // of the kernel that setting runs (split forms: the Gaussian / uniform-radius kernel and the
+  // two-node kernel build them).
+  std::vector<std::pair<int, float>> cand;
+  if (binned)
+    cand = {{2, 1.005f}, {2, 1.01f}, {2, 1.02f}, {3, 1.03f}, {3, 1.04f}, {3, 1.06f}, {4, 1.1f}, {4, 1.15f},
+            {5, 1.2f}, {5, 1.3f}};
+  else if (gauss)
+    cand = {{nodes0, 1.25f}, {nodes0, 1.5f}, {nodes0, 2.0f}, {nodes0, 4.0f}};
+  else
+    cand = {{nodes0, ratio0}};
+  c->number_of_h_bins = 0;
+  for (size_t k = 0; ok && k < cand.size(); ++k) consider(cand[k].first, cand[k].second, tile0);
+  const bool tiles = form != VIS_FORM_MFMA_F16 && (!binned || best_nodes == 2);
+  for (int t = VIS_TILE_DEFAULT; ok && tiles && t <= VIS_TILE_QUARTER; ++t)
+    if (t != tile0) consider(best_nodes, best_ratio, t);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->number_of_h_bins = bins0;
   c->number_of_interpolation_nodes = nodes0;
   c->h_bin_ratio = ratio0;
+  c->tile = tile0;
   if (!ok) return -1.0;
   c->number_of_h_bins = 0;
   c->number_of_interpolation_nodes = best_nodes;
   c->h_bin_ratio = best_ratio;
+  c->tile = best_tile;
   return best_ms;
 }
 

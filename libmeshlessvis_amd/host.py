@@ -89,6 +89,7 @@ class Config:
         nodes=0,
         h_bin_ratio=0.0,
         cull=False,
+        tile=0,
     ):
         L = lib()
         L.vis_clear_last_error()
@@ -111,6 +112,7 @@ class Config:
         self.cs.number_of_h_bins = int(h_bins)
         self.cs.number_of_interpolation_nodes = int(nodes)
         self.cs.h_bin_ratio = float(h_bin_ratio)
+        self.cs.tile = abi.TILE[tile] if isinstance(tile, str) else int(tile)
         self.cs.cull_fully_aliased_terms = bool(cull)
 
     # -- fields ------------------------------------------------------------
@@ -175,7 +177,7 @@ class Config:
         check("fourier_transform")
 
     def autotune(self, ds: Dataset, tolerance: float = 1e-5) -> float:
-        """vis_autotune: choose nodes per bin and bin ratio for the MFMA form on ``ds``
+        """vis_autotune: choose nodes per bin, bin ratio and tile for the MFMA form on ``ds``
         (registered); returns the chosen setting's spectrum time in ms."""
         ms = lib().vis_autotune(ds.ptr, self.p, float(tolerance))
         check("vis_autotune")

@@ -51,7 +51,7 @@ int main() {
   F(VisConfig, cull_fully_aliased_terms) F(VisConfig, _automatic_d_image) F(VisConfig, _plan)
   F(VisConfig, _number_of_samples) F(VisConfig, block_length) F(VisConfig, _scale) F(VisConfig, _d_image)
   F(VisConfig, _d_freq_image) F(VisConfig, _number_of_partial_sums) F(VisConfig, _d_freq_image_arranged)
-  F(VisConfig, form) F(VisConfig, phi_mode) F(VisConfig, number_of_h_bins) F(VisConfig, number_of_interpolation_nodes) F(VisConfig, h_bin_ratio)
+  F(VisConfig, form) F(VisConfig, phi_mode) F(VisConfig, number_of_h_bins) F(VisConfig, number_of_interpolation_nodes) F(VisConfig, h_bin_ratio) F(VisConfig, tile)
   return 0;
 }
 """

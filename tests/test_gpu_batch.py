@@ -32,7 +32,9 @@ def make_sets(rng):
     return sets
 
 
-def test_batched_graph_replays_match_eager():
+@pytest.mark.parametrize("tile", ["default", "quarter"])
+def test_batched_graph_replays_match_eager(tile):
+    print(f"[case] tile {tile}", flush=True)
     rng = np.random.default_rng(21)
     sets = make_sets(rng)
     vws = views(rng, 2)
@@ -40,7 +42,7 @@ def test_batched_graph_replays_match_eager():
     for ds in sets:
         for vw in vws:
             cfg = host.Config(cutoff=(64, 32), samples=(128, 96), step=(0.5, 0.5), u=vw.u, v=vw.v,
-                              form="mfma_split_f16", phi_mode="analytic")
+                              form="mfma_split_f16", phi_mode="analytic", tile=tile)
             pairs.append(ds)
             cfgs.append(cfg)
     for ds in sets:

@@ -1,8 +1,11 @@
 """Spectrum shapes at the edges of the MFMA kernels' tiling, against the FP64 oracle:
 cutoffs that leave partial |x| and y tiles (sum kernel: 128 x 128 tiles; two-node
 binned kernel: 64 x 64; three-node kernel: 128 x-columns x 64), a single |x| column,
-and an empty half plane (cutoff 0).  Block length 64 lets 2cx*cy take these values.
-Tolerance as the other split-FP16 tests: relative L2 <= 1e-5."""
+and an empty half plane (cutoff 0), for the default tiles and every VisTile variant
+(half the |x| rows, half the y columns, both).  Block length 64 lets 2cx*cy take these
+values.  Tolerance as the other split-FP16 tests: relative L2 <= 1e-5 (split BF16:
+3e-5); a variant against the default tile: <= 1e-6 (the tiles cut the particles into
+other work items, so the FP32 partial sums end at other particles)."""
 import numpy as np
 import pytest
 
@@ -21,10 +24,10 @@ def samples_for(cutoff):
     return (1 << (m - 1).bit_length(),) * 2
 
 
-def render(groups, cutoff, nodes):
+def render(groups, cutoff, nodes, tile="default", form="mfma_split_f16"):
     ds = host.Dataset(groups)
     with host.Config(cutoff=cutoff, samples=samples_for(cutoff), step=STEP, u=U, v=V, block_length=64,
-                     form="mfma_split_f16", phi_mode="analytic", nodes=nodes) as cfg:
+                     form=form, phi_mode="analytic", nodes=nodes, tile=tile) as cfg:
         assert cfg.check()
         cfg.register(ds)
         try:
@@ -71,3 +74,59 @@ def test_empty_half_plane(kind):
     F, img = render(groups, (0, 8), nodes)
     assert F.size == 0
     assert not img.any()
+
+
+TILED = {
+    "gauss_var": ("GAUSSIAN", 0, "var"),               # sum kernel, Gaussian bins
+    "uniform_wendland": ("WENDLAND_D3_C2", 0, "const"),  # sum kernel, uniform radius
+    "wendland_t2": ("WENDLAND_D3_C2", 2, "var"),        # two-node binned kernel
+}
+
+
+@pytest.mark.parametrize("tile", ["half_x", "half_y", "quarter"])
+@pytest.mark.parametrize("kind", list(TILED))
+@pytest.mark.parametrize("cutoff", [(1, 32), (80, 70), (200, 40)])
+def test_tile_variants(kind, tile, cutoff):
+    basis, nodes, radii = TILED[kind]
+    rng = np.random.default_rng(sum(cutoff) + 11)
+    n = 2500
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    w = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    if radii == "var":
+        h = np.exp(rng.uniform(np.log(0.01), np.log(0.08), n)).astype(np.float32)
+    else:
+        h = np.full(n, 0.03, np.float32)
+    groups = [(pos, w, h, basis)]
+    print(f"[case] tile {tile} {kind} cutoff={cutoff}", flush=True)
+    F, img = render(groups, cutoff, nodes, tile=tile)
+    F_def, _ = render(groups, cutoff, nodes)
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"rel-L2 {e:.2e} vs FP64, {oracle.rel_l2(F, F_def):.2e} vs the default tile")
+    assert e <= 1e-5
+    assert oracle.rel_l2(F, F_def) <= 1e-6
+    img64 = oracle.c2r64(oracle.arrange64(F64, *cutoff, *samples_for(cutoff)), *samples_for(cutoff))
+    assert oracle.rel_l2(img, img64) <= 1e-5
+
+
+@pytest.mark.parametrize("tile", ["half_x", "half_y", "quarter"])
+@pytest.mark.parametrize("kind", ["gauss_var", "wendland_t2"])
+def test_tile_variants_bf16(kind, tile):
+    basis, nodes, _ = TILED[kind]
+    rng = np.random.default_rng(5)
+    n = 2000
+    groups = [(rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32), rng.uniform(-1.0, 1.0, n).astype(np.float32),
+               np.exp(rng.uniform(np.log(0.01), np.log(0.08), n)).astype(np.float32), basis)]
+    cutoff = (80, 70)
+    print(f"[case] bf16 tile {tile} {kind}", flush=True)
+    F, _ = render(groups, cutoff, nodes, tile=tile, form="mfma_split_bf16")
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"rel-L2 {e:.2e}")
+    assert e <= 3e-5
+
+
+def test_unknown_tile_is_refused():
+    with host.Config(cutoff=(8, 8), samples=(64, 64), block_length=64, form="mfma_split_f16") as cfg:
+        cfg.cs.tile = 4
+        assert not cfg.check()

@@ -157,8 +157,9 @@ def test_autotune_meets_tolerance(basis):
         cfg.render(ds)
         e = oracle.rel_l2(cfg.spectrum(), F64)
         print(f"autotune {basis}: T={cfg.cs.number_of_interpolation_nodes} ratio={cfg.cs.h_bin_ratio:.3f} "
-              f"{ms:.3f} ms rel-L2 {e:.3e}")
+              f"tile={cfg.cs.tile} {ms:.3f} ms rel-L2 {e:.3e}")
         assert e <= 1.2e-5
+        assert 0 <= cfg.cs.tile <= 3
         # An unreachable tolerance keeps the most accurate candidate, still a valid setting.
         assert cfg.autotune(ds, 1e-12) > 0.0
         cfg.render(ds)
@@ -202,3 +203,26 @@ def test_renders_are_deterministic(form, basis, radii_kind, nodes):
             cfg.unregister(ds)
     for F, img in out[1:]:
         assert np.array_equal(F, out[0][0]) and np.array_equal(img, out[0][1])
+
+
+def test_autotune_tile_on_a_small_view():
+    """On a 4096-particle view with a 64 x 64 image (cutoff 32: a 32 x 33 half plane) the default
+    128 x 128 tile of the sum kernel is mostly empty; vis_autotune times every VisTile variant and
+    keeps the fastest within the tolerance.  The kept tile is a valid variant and its render is
+    within 1e-5 of the FP64 oracle."""
+    wl = W.uniform_gaussian_4096()
+    vw = wl.views[0]
+    ds = host.Dataset([wl.group()])
+    F64 = oracle.spectrum64([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, wl.phi_mode)
+    with host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form="mfma_split_f16",
+                     phi_mode=wl.phi_mode) as cfg:
+        cfg.register(ds)
+        default_ms = cfg.autotune(ds, 1e-5)  # warms the kernels; then time once more
+        cfg.cs.tile = 0
+        ms = cfg.autotune(ds, 1e-5)
+        cfg.render(ds)
+        e = oracle.rel_l2(cfg.spectrum(), F64)
+        print(f"tile {cfg.cs.tile}: {ms:.4f} ms (first pass {default_ms:.4f} ms), rel-L2 {e:.2e}")
+        assert 0 <= cfg.cs.tile <= 3
+        assert e <= 1e-5
+        cfg.unregister(ds)
