@@ -54,6 +54,8 @@ def make_workload(name: str, n: int | None):
         return W.uniform_signed_gaussian(n or 1_048_576)
     if name == "uniform_gaussian_4096":
         return W.uniform_gaussian_4096()
+    if name in ("timing_test", "timing_test_factor1"):
+        return W.timing_test(n or 1_048_576, factor=1 if name.endswith("factor1") else 0)
     raise SystemExit(f"unknown workload {name}")
 
 

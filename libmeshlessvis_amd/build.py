@@ -4,7 +4,8 @@
 
 Every library source under csrc/ is compiled for gfx950 into _build/ at the
 repository root, then linked into libmeshlessvis_amd/libmeshless_vis.so; the batch
-renderer csrc/vis_noninteractive.cpp becomes libmeshlessvis_amd/vis_noninteractive.  The FP32
+renderer csrc/vis_noninteractive.cpp and the timing harness csrc/vis_timing_test.cpp become
+libmeshlessvis_amd/vis_noninteractive and libmeshlessvis_amd/vis_timing_test.  The FP32
 packed-math feature is switched off for device code (the gfx950 packed FP32
 instructions are kept out of every kernel).
 """
@@ -23,6 +24,7 @@ CSRC = PKG / "csrc"
 BUILD = ROOT / "_build" / "libmeshless_vis"
 LIB = PKG / "libmeshless_vis.so"
 CLI = PKG / "vis_noninteractive"
+PROGRAMS = {"vis_noninteractive": CLI, "vis_timing_test": PKG / "vis_timing_test"}
 ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
 
 SOURCES = ["vis_runtime.cpp", "errors.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
@@ -87,16 +89,18 @@ def build(verbose: bool = True) -> Path:
 
 
 def build_cli(verbose: bool = True) -> Path:
-    """The batch renderer (vis_noninteractive.cpp), linked against the library beside it."""
-    tmp = CLI.with_suffix(".tmp")
-    cmd = [_hipcc(), "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / "vis_noninteractive.cpp"),
-           f"-L{PKG}", "-lmeshless_vis", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for vis_noninteractive.cpp:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, CLI)
-    if verbose:
-        print(f"[build] {CLI}")
+    """The programs on the C ABI - the batch renderer (vis_noninteractive.cpp) and the timing
+    harness (vis_timing_test.cpp) - linked against the library beside them."""
+    for name, out in PROGRAMS.items():
+        tmp = out.with_suffix(".tmp")
+        cmd = [_hipcc(), "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(CSRC / f"{name}.cpp"),
+               f"-L{PKG}", "-lmeshless_vis", "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {name}.cpp:\n{' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+        os.replace(tmp, out)
+        if verbose:
+            print(f"[build] {out}")
     return CLI
 
 

@@ -99,6 +99,17 @@ def uniform_gaussian_4096(seed: int = 1) -> Workload:
                     (32, 32), (64, 64), (1.0, 1.0), [View((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))], "reference")
 
 
+def timing_test(n: int = 1_048_576, cutoff: int = 128, samples: int = 256, factor: int = 0) -> Workload:
+    """The reference's timing harness (vis_timing_test/main.cpp:68-78): every particle at the
+    origin with weight 1, SPH basis, radius 1e-5 (factor 0) or no radii (factor 1), step 1, the
+    axis-aligned view, a square cutoff and image; Phi in the reference's formulas."""
+    pos = np.zeros((n, 3), np.float32)
+    w = np.ones(n, np.float32)
+    radii = None if factor else np.full(n, 1e-5, np.float32)
+    return Workload(f"timing_test_{n}_factor{factor}", pos, w, radii, "SPH", (cutoff, cutoff), (samples, samples),
+                    (1.0, 1.0), [View((1.0, 0.0, 0.0), (0.0, 1.0, 0.0))], "reference")
+
+
 def clustered_wendland(n: int = 262_144, seed: int = 2, workers: int = -1) -> Workload:
     rng = np.random.default_rng(seed)
     centres = rng.uniform(0.2, 0.8, size=(8, 3))
