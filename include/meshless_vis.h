@@ -22,8 +22,9 @@
  *    block_length (fourier_transform.cu:167-171).
  *  - vis_config_manual_d_image takes float*, as the reference's definitions
  *    do (meshless_vis.cu:244); its header said float2*.
- *  - vis_opengl_fourier_volume_rendering has no GL interop on a headless
- *    build: it records VIS_ERROR_UNSUPPORTED and does nothing.
+ *  - vis_opengl_fourier_volume_rendering renders through HIP's GL interop
+ *    into the buffer object and then restores the config's own _d_image (the
+ *    reference left _d_image pointing into the unmapped buffer).
  *  - fields after _d_freq_image_arranged are additive (form, phi_mode, ...).
  */
 #ifndef MESHLESS_VISh
@@ -144,7 +145,10 @@ void vis_config_compute_scale(VisConfig* vis_config);
 /* meshless_vis.h:90-91 / meshless_vis.cu:264-299: H2D copy, zero padding to a multiple of P*block_length */
 void vis_register_meshless_dataset(VisConfig* vis_config, MeshlessDataset* meshless_dataset);
 void vis_unregister_meshless_dataset(VisConfig* vis_config, MeshlessDataset* meshless_dataset);
-/* meshless_vis.h:92 / meshless_vis.cu:357-364: unsupported here (records VIS_ERROR_UNSUPPORTED). */
+/* meshless_vis.h:92 / meshless_vis.cu:357-364: registers and maps the GL buffer object
+   (hipGraphicsGLRegisterBuffer, on the config's stream), renders the image into it (at least
+   Nx*Ny floats, else VIS_ERROR_INVALID_VALUE), unmaps and unregisters it.  Needs a current GL
+   context on the GPU; interop failures are reported as VIS_ERROR_HIP. */
 void vis_opengl_fourier_volume_rendering(MeshlessDataset* meshless_dataset, VisConfig* vis_config, unsigned int buffer_object);
 /* meshless_vis.h:93 / meshless_vis.cu:334-350: asynchronous on the config's stream. */
 void vis_fourier_volume_rendering(MeshlessDataset* meshless_dataset, VisConfig* vis_config);

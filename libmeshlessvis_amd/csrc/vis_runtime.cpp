@@ -8,6 +8,8 @@
 #include "internal.h"
 #include "mfma.h"
 
+#include <hip/hip_gl_interop.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1031,8 +1033,54 @@ void vis_fourier_volume_rendering_noalloc(MeshlessDataset* ds, VisConfig* c) {
   render_noalloc(ds, c);
 }
 
-void vis_opengl_fourier_volume_rendering(MeshlessDataset*, VisConfig*, unsigned int) {
-  set_error(VIS_ERROR_UNSUPPORTED, "vis_opengl_fourier_volume_rendering: no OpenGL interop in this build");
+// The reference (meshless_vis.cu:357-364) registers the GL buffer object with CUDA, maps it,
+// points _d_image at it, renders, unmaps and unregisters.  Here through HIP's GL interop, on
+// the config's stream: the buffer is mapped on the stream (GL work on it is ordered before the
+// render), the render writes the image into it, and the unmap orders the render before later GL
+// work.  Deliberate difference: the config's own _d_image is restored afterwards (the
+// reference left it pointing into the unmapped buffer).  The buffer must hold at least
+// Nx * Ny floats.  A failure of the interop calls (no current GL context, a name that is not a
+// buffer object) is reported as VIS_ERROR_HIP and renders nothing.
+void vis_opengl_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c, unsigned int buffer_object) {
+  clear_error();
+  if (!ds || !c) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_opengl_fourier_volume_rendering: null argument");
+    return;
+  }
+  std::string why;
+  if (!config_valid(c, &why)) {
+    set_error(VIS_ERROR_INVALID_CONFIG, "invalid VisConfig: " + why);
+    return;
+  }
+  hipStream_t s = c->_plan->stream;
+  hipGraphicsResource* res = nullptr;
+  if (!hip_ok(hipGraphicsGLRegisterBuffer(&res, buffer_object, hipGraphicsRegisterFlagsWriteDiscard),
+              "hipGraphicsGLRegisterBuffer"))
+    return;
+  auto unregister = [&] { (void)hipGraphicsUnregisterResource(res); };
+  if (!hip_ok(hipGraphicsMapResources(1, &res, s), "hipGraphicsMapResources")) {
+    unregister();
+    return;
+  }
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  const size_t need = sizeof(float) * static_cast<size_t>(c->_number_of_samples.x) * c->_number_of_samples.y;
+  bool ok = hip_ok(hipGraphicsResourceGetMappedPointer(&ptr, &bytes, res), "hipGraphicsResourceGetMappedPointer");
+  if (ok && bytes < need) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_opengl_fourier_volume_rendering: buffer object smaller than the image");
+    ok = false;
+  }
+  if (ok) {
+    float* own = c->_d_image;
+    c->_d_image = static_cast<float*>(ptr);
+    vis_fourier_volume_rendering(ds, c);
+    c->_d_image = own;
+    ok = vis_get_last_error_code() == VIS_SUCCESS;
+  }
+  // Unmap and unregister whatever happened; the first error stays the one reported.
+  const hipError_t e1 = hipGraphicsUnmapResources(1, &res, s);
+  const hipError_t e2 = hipGraphicsUnregisterResource(res);
+  if (ok) hip_ok(e1 != hipSuccess ? e1 : e2, "hipGraphicsUnmapResources / hipGraphicsUnregisterResource");
 }
 
 void vis_copy_to_host(VisConfig* c, float* h_image) {

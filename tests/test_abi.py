@@ -91,7 +91,7 @@ def test_failures_are_error_codes_not_aborts():
     L.vis_fourier_volume_rendering(None, None)
     assert L.vis_get_last_error_code() == 1
     L.vis_opengl_fourier_volume_rendering(None, None, 0)
-    assert L.vis_get_last_error_code() == 5
+    assert L.vis_get_last_error_code() == 1
     L.vis_clear_last_error()
 
 
