@@ -28,6 +28,12 @@ PROGRAMS = {"vis_noninteractive": CLI, "vis_timing_test": PKG / "vis_timing_test
 ARCH = os.environ.get("MESHLESS_VIS_ARCH", "gfx950")
 
 SOURCES = ["vis_runtime.cpp", "errors.cpp", "meshless.cpp", "prep.hip", "direct_sum.hip", "fft.hip", "mfma.hip", "mfma_binned.hip"]
+# fft_kernels.hip is compiled once per C2R kernel variant (pass, odd radices, largest block), each
+# object holding one kernel, so that the variants compile in parallel.
+FFT_VARIANTS = [(p, o, t) for p in (0, 1) for o in (0, 1) for t in (512, 1024)]
+UNITS = [(name, name, []) for name in SOURCES] + [
+    ("fft_kernels.hip", f"fft_kernels.{'xy'[p]}{o}_{t}.hip",
+     [f"-DMV_FFT_PASS={p}", f"-DMV_FFT_ODD={o}", f"-DMV_FFT_MAXT={t}"]) for p, o, t in FFT_VARIANTS]
 
 
 def _hipcc() -> str:
@@ -50,8 +56,8 @@ def _common_flags() -> list[str]:
     ]
 
 
-def _compile(src: Path, obj: Path) -> None:
-    cmd = [_hipcc(), *_common_flags()]
+def _compile(src: Path, obj: Path, defines=()) -> None:
+    cmd = [_hipcc(), *_common_flags(), *defines]
     if src.suffix == ".cpp":
         cmd += ["-x", "hip"]
     cmd += ["-c", str(src), "-o", str(obj)]
@@ -68,12 +74,13 @@ def build(verbose: bool = True) -> Path:
     BUILD.mkdir(parents=True, exist_ok=True)
     objs = []
     jobs = []
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        for name in SOURCES:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(UNITS))) as ex:
+        # The slowest units (the C2R kernels, the binned MFMA kernels) are queued first.
+        for name, unit, defines in sorted(UNITS, key=lambda u: not u[0].startswith(("fft_kernels", "mfma"))):
             src = CSRC / name
-            obj = BUILD / (name + ".o")
+            obj = BUILD / (unit + ".o")
             objs.append(obj)
-            jobs.append(ex.submit(_compile, src, obj))
+            jobs.append(ex.submit(_compile, src, obj, defines))
         for j in jobs:
             j.result()
     tmp = LIB.with_suffix(".so.tmp")

@@ -108,12 +108,15 @@ __device__ __forceinline__ double2 sum_chunks_block(const float2* __restrict__ p
 // cull > 0: a particle whose projected footprint, the square of half-side cull * r0
 // around (p.u, p.v), lies wholly outside the image's period window
 // [-1/(2 sx), 1/(2 sx)] x [-1/(2 sy), 1/(2 sy)] (a fully aliased term) gets weight 0.
+// zero, nzero (at most 256): ints the launch also clears (the work-queue counters of the MFMA
+// launches that follow); nullptr, 0 for none.
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
-                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s);
+                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s,
+                               int* zero = nullptr, int nzero = 0);
 
 hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
                                       int n_slots, double sx, double sy, double3 u, double3 v, float cull,
-                                      Packet* out, hipStream_t s);
+                                      Packet* out, hipStream_t s, int* zero = nullptr, int nzero = 0);
 
 // The cull factor of a group under a config: 0 unless cull_fully_aliased_terms is set;
 // else the basis's support in units of r0 (SPH 2, Wendland 1, Gaussian 3: exp(-9 pi)).
@@ -133,7 +136,8 @@ struct PrepJob {
 };
 // Every job's packets in one launch; begin[j] = first flattened slot of job j (prefix sums of
 // n_slots, ascending), total = all slots.
-hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s);
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s,
+                            int* zero = nullptr, int nzero = 0);
 
 // One render of a batch after the MFMA launches: its jobs' split-K partials
 // (JobPart[job_begin, job_end), summed in FP64 in a fixed order) give F = scale * (re, -im);
@@ -193,5 +197,10 @@ hipError_t launch_zero_words(void* p, size_t words, hipStream_t s);
 // Nx*(Ny/2+1) complex buffer (_d_freq_image_arranged), used as the intermediate [cy][Nx].
 hipError_t launch_c2r(const float2* F, int cx, int cy, int Nx, int Ny, const FftPlan& plan, float2* work,
                       float* image, hipStream_t s);
+// The finalize and the C2R of a single render in the two C2R launches: the first pass writes the
+// spectrum F = scale * (re, -im) of the FP64 sums acc[cy][2cx] as it gathers it (the same values as
+// launch_finalize_spectrum) and transforms it.
+hipError_t launch_finalize_c2r(const double2* acc, float scale, float2* F, int cx, int cy, int Nx, int Ny,
+                               const FftPlan& plan, float2* work, float* image, hipStream_t s);
 
 }  // namespace mv

@@ -63,9 +63,10 @@ struct MfmaWorkspace {
 bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, MfmaBatch* bt);
 // Launches a prepared batch: packet prep, persistent MFMA kernels, reductions (graph-capturable).
 bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev);
-// Only the work-queue reset and the persistent MFMA kernels (packets prepared, partials
-// reduced by the caller).
-bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev);
+// Only the work-queue reset (unless a prep launch cleared the counters) and the persistent MFMA
+// kernels (packets prepared, partials reduced by the caller).
+bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev,
+                      bool counters_cleared = false);
 // prepare + enqueue.
 bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
 

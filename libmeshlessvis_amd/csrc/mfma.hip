@@ -713,21 +713,28 @@ const int* mfma_slot_map(const MfmaPlanned& pl) { return pl.bc ? pl.bc->slot_map
 // Device half: kernel launches only (safe to capture into a HIP graph).
 bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
   if (specs.empty()) return true;
+  // The first packet prep also clears the work-queue counters (a prep launch that clears
+  // counters runs at least one block), one launch fewer per render.
+  const int nkeys = static_cast<int>(bt.keys.size());
+  const bool fused = nkeys <= 256;  // one block clears them
   for (size_t k = 0; k < specs.size(); ++k) {
     const MfmaJobSpec& sp = specs[k];
     const MfmaPlanned& pl = bt.plans[k];
+    int* zero = k == 0 && fused ? bt.counters : nullptr;
+    const int nzero = k == 0 && fused ? nkeys : 0;
     hipError_t e;
     if (pl.bc)
       e = launch_prep_packets_mapped(sp.g->d_constraints, sp.g->d_radii, pl.bc->slot_map.as<int>(), pl.n_slots,
                                      sp.c->step_size.x, sp.c->step_size.y, sp.u, sp.v,
-                                     cull_factor(sp.c, sp.g->basis_function_id), const_cast<Packet*>(pl.job.packets), s);
+                                     cull_factor(sp.c, sp.g->basis_function_id), const_cast<Packet*>(pl.job.packets), s,
+                                     zero, nzero);
     else
       e = launch_prep_packets(sp.g->d_constraints, sp.g->d_radii, sp.n, pl.n_slots, sp.c->step_size.x,
                               sp.c->step_size.y, sp.u, sp.v, cull_factor(sp.c, sp.g->basis_function_id),
-                              const_cast<Packet*>(pl.job.packets), s);
+                              const_cast<Packet*>(pl.job.packets), s, zero, nzero);
     if (!hip_ok(e, "prep_packets launch")) return false;
   }
-  if (!mfma_launch_sums(specs, bt, s, ev)) return false;
+  if (!mfma_launch_sums(specs, bt, s, ev, fused)) return false;
   for (size_t k = 0; k < specs.size(); ++k) {
     const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
     hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(256), 0, s, bt.plans[k].job.partial,
@@ -737,10 +744,11 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
   return true;
 }
 
-bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
+bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev,
+                      bool counters_cleared) {
   if (specs.empty()) return true;
   const VisConfig* c0 = specs[0].c;
-  if (!hip_ok(launch_zero_words(bt.counters, bt.keys.size(), s), "zero counters")) return false;
+  if (!counters_cleared && !hip_ok(launch_zero_words(bt.counters, bt.keys.size(), s), "zero counters")) return false;
 
   MfmaShared sh{};
   sh.cx = specs[0].vp.cx;

@@ -11,6 +11,8 @@
 // quantisation error is at most (|x|+|y|) * 2^-33 revolutions.
 #include "internal.h"
 
+#include <algorithm>
+
 namespace mv {
 
 __device__ __forceinline__ uint32_t to_fixed_rev(double a) {
@@ -48,10 +50,17 @@ __device__ __forceinline__ Packet make_packet(const Constraint* __restrict__ c, 
   return p;
 }
 
+// Block 0 also clears `nzero` (< 256) ints at `zero`: the work-queue counters of the MFMA launches
+// that follow, cleared here instead of by a launch of their own.
+__device__ __forceinline__ void clear_counters(int* zero, int nzero) {
+  if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < nzero) zero[threadIdx.x] = 0;
+}
+
 __global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __restrict__ c,
                                                            const float* __restrict__ radii, int n, int n_padded,
                                                            double sx, double sy, double3 u, double3 v, float cull,
-                                                           Packet* __restrict__ out) {
+                                                           Packet* __restrict__ out, int* zero, int nzero) {
+  clear_counters(zero, nzero);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_padded) return;
   out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v, cull);
@@ -62,7 +71,9 @@ __global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constrai
                                                                   const float* __restrict__ radii,
                                                                   const int* __restrict__ slot_map, int n_slots,
                                                                   double sx, double sy, double3 u, double3 v,
-                                                                  float cull, Packet* __restrict__ out) {
+                                                                  float cull, Packet* __restrict__ out, int* zero,
+                                                                  int nzero) {
+  clear_counters(zero, nzero);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n_slots) return;
   out[k] = make_packet(c, radii, slot_map[k], sx, sy, u, v, cull);
@@ -71,7 +82,8 @@ __global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constrai
 // All jobs of a batch: thread g of the flattened slot range finds its job by binary search.
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restrict__ jobs,
                                                         const long long* __restrict__ begin, int n_jobs,
-                                                        long long total) {
+                                                        long long total, int* zero, int nzero) {
+  clear_counters(zero, nzero);
   const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= total) return;
   int lo = 0, hi = n_jobs - 1;
@@ -89,29 +101,40 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restric
   J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v, J.cull);
 }
 
-hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s) {
-  if (n_jobs <= 0 || total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_jobs_kernel, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, jobs, begin,
-                     n_jobs, total);
+// Grid of a prep launch: a block per 256 slots, at least one when it clears counters.
+static unsigned prep_grid(long long items, int nzero) {
+  return static_cast<unsigned>(std::max<long long>(nzero > 0 ? 1 : 0, (items + 255) / 256));
+}
+
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s,
+                            int* zero, int nzero) {
+  if (nzero > 256) return hipErrorInvalidValue;
+  if (n_jobs <= 0) total = 0;
+  const unsigned grid = prep_grid(std::max(0LL, total), nzero);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_jobs_kernel, dim3(grid), dim3(256), 0, s, jobs, begin, n_jobs, total, zero, nzero);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
                                       int n_slots, double sx, double sy, double3 u, double3 v, float cull,
-                                      Packet* out, hipStream_t s) {
-  if (n_slots <= 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_packets_mapped_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, s, d_constraints, d_radii,
-                     slot_map, n_slots, sx, sy, u, v, cull, out);
+                                      Packet* out, hipStream_t s, int* zero, int nzero) {
+  if (nzero > 256) return hipErrorInvalidValue;
+  const unsigned grid = prep_grid(std::max(0, n_slots), nzero);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_packets_mapped_kernel, dim3(grid), dim3(256), 0, s, d_constraints, d_radii, slot_map,
+                     n_slots, sx, sy, u, v, cull, out, zero, nzero);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_radii, int n, int n_padded,
-                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s) {
-  if (n_padded <= 0) return hipSuccess;
-  const int block = 256;
-  const int grid = (n_padded + block - 1) / block;
-  hipLaunchKernelGGL(prep_packets_kernel, dim3(grid), dim3(block), 0, s, d_constraints, d_radii, n, n_padded, sx,
-                     sy, u, v, cull, out);
+                               double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s,
+                               int* zero, int nzero) {
+  if (nzero > 256) return hipErrorInvalidValue;
+  const unsigned grid = prep_grid(std::max(0, n_padded), nzero);
+  if (grid == 0) return hipSuccess;
+  hipLaunchKernelGGL(prep_packets_kernel, dim3(grid), dim3(256), 0, s, d_constraints, d_radii, n, n_padded, sx, sy,
+                     u, v, cull, out, zero, nzero);
   return hipGetLastError();
 }
 

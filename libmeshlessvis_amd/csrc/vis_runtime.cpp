@@ -380,8 +380,9 @@ static bool plain_group(VisPlan* p, const MfmaJobSpec& sp, int nfreq) {
                 "reduce_partials launch");
 }
 
-// The spectrum of all groups into _d_freq_image slot 0.
-static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
+// The spectrum of all groups: the FP64 sums into the plan's accumulator and, with finalize,
+// scaled into _d_freq_image slot 0 (without: the caller's first C2R pass does that).
+static bool spectrum(MeshlessDataset* ds, VisConfig* c, bool finalize = true) {
   VisPlan* p = c->_plan;
   const int nfreq = static_cast<int>(spectrum_count(c));
   if (nfreq == 0) return true;
@@ -394,6 +395,7 @@ static bool spectrum(MeshlessDataset* ds, VisConfig* c) {
     for (const MfmaJobSpec& sp : specs)
       if (!plain_group(p, sp, nfreq)) return false;
   }
+  if (!finalize) return true;
   return hip_ok(launch_finalize_spectrum(p->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, p->stream),
                 "finalize launch");
 }
@@ -583,8 +585,12 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
     // One launch each: every job's packets, the persistent MFMA kernels, every render's
     // reduction + finalize, and the two C2R passes of every render.
     const VisConfig* c0 = cf[0];
-    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, q), "prep launch") &&
-         mfma_launch_sums(specs, bg->ws.batch, q, nullptr) &&
+    const int nkeys = static_cast<int>(bg->ws.batch.keys.size());
+    const bool fused = nkeys <= 256;  // the prep launch clears the work-queue counters
+    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, q,
+                                 fused ? bg->ws.batch.counters : nullptr, fused ? nkeys : 0),
+                "prep launch") &&
+         mfma_launch_sums(specs, bg->ws.batch, q, nullptr, fused) &&
          hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, q),
                 "finish launch") &&
          hip_ok(launch_c2r_batch(outs, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, c0->_number_of_samples.x,
@@ -595,12 +601,16 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
     for (int i = 0; ok && i < n; ++i) {
       VisConfig* c = cf[i];
       const int nfreq = static_cast<int>(spectrum_count(c));
-      ok = hip_ok(launch_finalize_spectrum(c->_plan->acc.as<double2>(), nfreq, c->_scale, c->_d_freq_image, q),
-                  "finalize launch") &&
-           hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                             c->_number_of_samples.y, c->_plan->fft, c->_d_freq_image_arranged,
-                             c->_d_image, q),
-                  "c2r launch");
+      ok = nfreq == 0 || hip_ok(launch_finalize_c2r(c->_plan->acc.as<double2>(), c->_scale, c->_d_freq_image,
+                                                    c->_cutoff_frequency.x, c->_cutoff_frequency.y,
+                                                    c->_number_of_samples.x, c->_number_of_samples.y, c->_plan->fft,
+                                                    c->_d_freq_image_arranged, c->_d_image, q),
+                                "c2r launch");
+      if (ok && nfreq == 0)
+        ok = hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y,
+                               c->_number_of_samples.x, c->_number_of_samples.y, c->_plan->fft,
+                               c->_d_freq_image_arranged, c->_d_image, q),
+                    "c2r launch");
     }
   }
   if (!capture) return ok;
@@ -750,8 +760,11 @@ static bool render_noalloc(MeshlessDataset* ds, VisConfig* c) {
                   "reduce_partials launch"))
         return false;
     }
-    if (!hip_ok(launch_finalize_spectrum(acc, nfreq, c->_scale, c->_d_freq_image, p->stream), "finalize launch"))
-      return false;
+    // The first C2R pass finalizes the spectrum from acc as it gathers it.
+    return hip_ok(launch_finalize_c2r(acc, c->_scale, c->_d_freq_image, c->_cutoff_frequency.x,
+                                      c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y, p->fft,
+                                      c->_d_freq_image_arranged, c->_d_image, p->stream),
+                  "c2r launch");
   }
   return hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
                            c->_number_of_samples.y, p->fft, c->_d_freq_image_arranged, c->_d_image,
@@ -998,15 +1011,20 @@ void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
   }
   VisPlan* p = c->_plan;
   const size_t nfreq = spectrum_count(c);
-  if (ds->number_of_groups < 1) {
+  if (ds->number_of_groups < 1 || nfreq == 0) {
     // No groups: the reference leaves the spectrum as it was; an empty dataset renders zero here.
     if (!hip_ok(launch_zero_words(c->_d_freq_image, 2 * nfreq, p->stream), "zero spectrum")) return;
-  } else {
-    vis_config_compute_scale(c);
-    if (!spectrum(ds, c)) return;
+    hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
+                      c->_number_of_samples.y, p->fft, c->_d_freq_image_arranged, c->_d_image, p->stream),
+           "c2r launch");
+    return;
   }
-  hip_ok(launch_c2r(c->_d_freq_image, c->_cutoff_frequency.x, c->_cutoff_frequency.y, c->_number_of_samples.x,
-                    c->_number_of_samples.y, p->fft, c->_d_freq_image_arranged, c->_d_image, p->stream),
+  vis_config_compute_scale(c);
+  if (!spectrum(ds, c, false)) return;
+  // The first C2R pass finalizes the spectrum (scale, sign) from the FP64 sums as it gathers it.
+  hip_ok(launch_finalize_c2r(p->acc.as<double2>(), c->_scale, c->_d_freq_image, c->_cutoff_frequency.x,
+                             c->_cutoff_frequency.y, c->_number_of_samples.x, c->_number_of_samples.y, p->fft,
+                             c->_d_freq_image_arranged, c->_d_image, p->stream),
          "c2r launch");
 }
 
