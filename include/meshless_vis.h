@@ -221,6 +221,12 @@ void vis_set_kernel_timing(VisConfig* vis_config, int enable);
 int vis_get_kernel_timing(VisConfig* vis_config, double* total_ms);
 void vis_reset_kernel_timing(VisConfig* vis_config);
 
+/* Test support (no counterpart in the reference): after `after` (>= 0) more device
+   allocations succeed, every later one fails as VIS_ERROR_OUT_OF_MEMORY without calling
+   hipMalloc, until this is called with a negative value (the default).  For exercising
+   the library's out-of-memory paths; process-wide. */
+void vis_debug_fail_device_allocations(int after);
+
 /* Last error of the calling thread (VIS_SUCCESS / "" when none). */
 int vis_get_last_error_code(void);
 const char* vis_get_last_error(void);

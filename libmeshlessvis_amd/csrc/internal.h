@@ -17,6 +17,8 @@ namespace mv {
 
 // ---------------------------------------------------------------- errors ---
 bool hip_ok(hipError_t e, const char* what);  // records VIS_ERROR_HIP / OOM on failure
+// hipMalloc through the library's allocation-failure hook; false (error recorded) on failure.
+bool device_malloc(void** p, size_t bytes, const char* what);
 
 // ------------------------------------------------------- particle packets ---
 // Per-view, per-particle record written by the prep kernel and streamed by the

@@ -30,6 +30,26 @@ bool hip_ok(hipError_t e, const char* what) {
   return false;
 }
 
+// ---------------------------------------------------- device allocation ---
+// Every device allocation of the library goes through device_malloc, which also serves the
+// allocation-failure hook (vis_debug_fail_device_allocations): with the hook set to k >= 0, k
+// more allocations succeed and every one after them fails as out of memory, without calling
+// hipMalloc, until the hook is set to a negative value.
+static std::atomic<long long> g_fail_after{-1};
+
+bool device_malloc(void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  long long left = g_fail_after.load();
+  while (left >= 0) {
+    if (left == 0) {
+      set_error(VIS_ERROR_OUT_OF_MEMORY, std::string(what) + ": allocation failure (injected)");
+      return false;
+    }
+    if (g_fail_after.compare_exchange_weak(left, left - 1)) break;
+  }
+  return hip_ok(hipMalloc(p, bytes), what);
+}
+
 // -------------------------------------------------------------- registry ---
 static std::mutex g_reg_mu;
 static std::unordered_map<const void*, GroupInfo>& reg() {
@@ -92,7 +112,7 @@ bool DevBuf::reserve(size_t need) {
   release();
   if (need == 0) return true;
   void* p = nullptr;
-  if (!hip_ok(hipMalloc(&p, need), "hipMalloc(workspace)")) return false;
+  if (!device_malloc(&p, need, "hipMalloc(workspace)")) return false;
   ptr = p;
   bytes = need;
   return true;
@@ -187,22 +207,22 @@ static bool alloc_freq_image(VisConfig* c) {
   c->_d_freq_image = nullptr;
   const size_t n = spectrum_count(c) * c->_number_of_partial_sums;
   if (n == 0) return true;
-  return hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_freq_image), sizeof(float2) * n), "hipMalloc(_d_freq_image)");
+  return device_malloc(reinterpret_cast<void**>(&c->_d_freq_image), sizeof(float2) * n, "hipMalloc(_d_freq_image)");
 }
 
 static bool alloc_samples(VisConfig* c) {
   if (c->_d_freq_image_arranged) (void)hipFree(c->_d_freq_image_arranged);
   c->_d_freq_image_arranged = nullptr;
   const size_t na = static_cast<size_t>(c->_number_of_samples.x) * (c->_number_of_samples.y / 2 + 1);
-  if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_freq_image_arranged), sizeof(float2) * std::max<size_t>(na, 1)),
-              "hipMalloc(_d_freq_image_arranged)"))
+  if (!device_malloc(reinterpret_cast<void**>(&c->_d_freq_image_arranged), sizeof(float2) * std::max<size_t>(na, 1),
+                     "hipMalloc(_d_freq_image_arranged)"))
     return false;
   if (c->_automatic_d_image) {
     if (c->_d_image) (void)hipFree(c->_d_image);
     c->_d_image = nullptr;
     const size_t ni = static_cast<size_t>(c->_number_of_samples.x) * c->_number_of_samples.y;
-    if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&c->_d_image), sizeof(float) * std::max<size_t>(ni, 1)),
-                "hipMalloc(_d_image)"))
+    if (!device_malloc(reinterpret_cast<void**>(&c->_d_image), sizeof(float) * std::max<size_t>(ni, 1),
+                       "hipMalloc(_d_image)"))
       return false;
   }
   return plan_set_samples(c->_plan, c->_number_of_samples.x, c->_number_of_samples.y);
@@ -248,6 +268,9 @@ static bool config_valid(const VisConfig* c, std::string* why) {
   if (c->_number_of_partial_sums < 1) { *why = "number_of_partial_sums < 1"; return false; }
   // Limits of this build (not in the reference).
   if (c->_cutoff_frequency.x < 0 || c->_cutoff_frequency.y < 0) { *why = "negative cutoff"; return false; }
+  // Buffers a failed allocation (vis_config_change_*) left missing.
+  if (spectrum_count(c) > 0 && !c->_d_freq_image) { *why = "no spectrum buffer (its allocation failed)"; return false; }
+  if (!c->_d_freq_image_arranged) { *why = "no FFT work buffer (its allocation failed)"; return false; }
   if (!fft_length_supported(c->_number_of_samples.x, why) || !fft_length_supported(c->_number_of_samples.y, why))
     return false;
   if (c->_plan->fft.x.N != c->_number_of_samples.x || c->_plan->fft.y.N != c->_number_of_samples.y) {
@@ -903,14 +926,35 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
     return;
   hipStream_t s = async ? p->copy_stream : p->stream;
   std::vector<std::pair<const void*, GroupInfo>> infos;
+  int processed = 0;  // groups whose device pointers this call has set
+  // On any failure the registration is undone: the copies already queued are waited for, every
+  // device array this call allocated is freed (and any bins built for it dropped), and those
+  // groups' device pointers are cleared (the first error stays the one reported).
+  auto rollback = [&]() {
+    const int code = vis_get_last_error_code();
+    const std::string msg = vis_get_last_error();
+    (void)hipStreamSynchronize(s);
+    for (int j = 0; j < processed; ++j) {
+      Group& g = ds->groups[j];
+      if (g.d_constraints) {
+        mfma_forget_group(g.d_constraints);
+        (void)hipFree(g.d_constraints);
+      }
+      if (g.d_radii) (void)hipFree(g.d_radii);
+      g.d_constraints = nullptr;
+      g.d_radii = nullptr;
+    }
+    set_error(code, msg);
+  };
   for (int j = 0; j < ds->number_of_groups; ++j) {
     Group& g = ds->groups[j];
     g.d_constraints = nullptr;
     g.d_radii = nullptr;
+    processed = j + 1;
     const int n = g.number_of_terms;
     if (n < 0) {
       set_error(VIS_ERROR_INVALID_VALUE, "negative number_of_terms");
-      return;
+      return rollback();
     }
     const long long rounded = ((n + multiple - 1) / multiple) * multiple;
     g.d_number_of_terms = static_cast<int>(rounded);
@@ -919,36 +963,38 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
     info.padded = static_cast<int>(rounded);
     group_stats(g.h_constraints, g.h_radii, n, &info);
     if (g.h_constraints && rounded > 0) {
-      if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_constraints), sizeof(Constraint) * rounded), "hipMalloc(d_constraints)"))
-        return;
+      if (!device_malloc(reinterpret_cast<void**>(&g.d_constraints), sizeof(Constraint) * rounded,
+                         "hipMalloc(d_constraints)"))
+        return rollback();
       if (!hip_ok(hipMemsetAsync(g.d_constraints, 0, sizeof(Constraint) * rounded, s), "hipMemset") ||
           !hip_ok(hipMemcpyAsync(g.d_constraints, g.h_constraints, sizeof(Constraint) * n, hipMemcpyHostToDevice, s),
                   "hipMemcpy(d_constraints)"))
-        return;
+        return rollback();
     }
     if (g.h_radii && rounded > 0) {
-      if (!hip_ok(hipMalloc(reinterpret_cast<void**>(&g.d_radii), sizeof(float) * rounded), "hipMalloc(d_radii)")) return;
+      if (!device_malloc(reinterpret_cast<void**>(&g.d_radii), sizeof(float) * rounded, "hipMalloc(d_radii)"))
+        return rollback();
       if (!hip_ok(hipMemsetAsync(g.d_radii, 0, sizeof(float) * rounded, s), "hipMemset") ||
           !hip_ok(hipMemcpyAsync(g.d_radii, g.h_radii, sizeof(float) * n, hipMemcpyHostToDevice, s), "hipMemcpy(d_radii)"))
-        return;
+        return rollback();
     }
     if (g.d_constraints) {
       infos.emplace_back(g.d_constraints, info);
       // The config's MFMA form bins the group from the host arrays now, on the same stream as
       // the copies: the first render needs no copy back from the device, and with asynchronous
       // registration this host work overlaps the work already queued.
-      if (!mfma_prebin(g, info, c, s)) return;
+      if (!mfma_prebin(g, info, c, s)) return rollback();
     }
   }
   if (async) {
     hipEvent_t ev = nullptr;
-    if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate")) return;
+    if (!hip_ok(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate")) return rollback();
     const bool ok = hip_ok(hipEventRecord(ev, s), "hipEventRecord") &&
                     hip_ok(hipStreamWaitEvent(p->stream, ev, 0), "hipStreamWaitEvent");
     (void)hipEventDestroy(ev);
-    if (!ok) return;
+    if (!ok) return rollback();
   } else if (!hip_ok(hipStreamSynchronize(s), "register synchronize")) {
-    return;
+    return rollback();
   }
   for (const auto& e : infos) registry_put(e.first, e.second);
 }
@@ -1279,6 +1325,8 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
   c->tile = best_tile;
   return best_ms;
 }
+
+void vis_debug_fail_device_allocations(int after) { g_fail_after.store(after < 0 ? -1 : after); }
 
 void vis_set_kernel_timing(VisConfig* c, int enable) {
   if (!c) return;
