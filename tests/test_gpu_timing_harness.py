@@ -6,6 +6,9 @@
   radius 1e-5 or none, step 1, the axis-aligned view.  All phases are 0, so the spectrum is
   N * Phi(|k| r0) (reference Phi formulas), checked against the FP64 oracle (relative L2
   <= 1e-5, split BF16 3e-5, as the other parity tests) and against that closed form.
+* Reference parity on the same dataset: the FP32 restatement of fourier_transform_cpu.cpp
+  (e_ref against FP64), the GPU (e_build) and the GPU against the restatement (e_par), with
+  e_par <= e_ref + e_build + 1e-6 as in the full-size parity tests.
 """
 import re
 import subprocess
@@ -75,3 +78,24 @@ def test_dataset_parity(form, tol, factor):
     assert e <= tol
     img64 = oracle.c2r64(oracle.arrange64(F64, *wl.cutoff, *wl.samples), *wl.samples)
     assert oracle.rel_l2(img, img64) <= tol
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma_split_f16"])
+@pytest.mark.parametrize("factor", [0, 1])
+def test_dataset_reference_parity(form, factor):
+    print(f"[case] harness reference parity {form} factor={factor}", flush=True)
+    wl = W.timing_test(4096, cutoff=32, samples=64, factor=factor)
+    vw = wl.views[0]
+    ds = host.Dataset([wl.group()])
+    with host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
+                     phi_mode=wl.phi_mode, block_length=64) as cfg:
+        cfg.register(ds)
+        cfg.render(ds)
+        F = cfg.spectrum()
+        cfg.unregister(ds)
+    F64 = oracle.spectrum64([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, wl.phi_mode)
+    Fport = oracle.port_spectrum32([wl.group()], wl.cutoff, wl.step, vw.u, vw.v, block_length=64,
+                                   phi_mode=wl.phi_mode)
+    e_ref, e_build, e_par = oracle.rel_l2(Fport, F64), oracle.rel_l2(F, F64), oracle.rel_l2(F, Fport)
+    print(f"e_ref {e_ref:.2e} e_build {e_build:.2e} e_par {e_par:.2e}")
+    assert e_par <= e_ref + e_build + 1e-6
