@@ -9,6 +9,7 @@
 #include "mfma.h"
 
 #include <hip/hip_gl_interop.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
 #include <cmath>
@@ -29,6 +30,19 @@ bool hip_ok(hipError_t e, const char* what) {
   (void)hipGetLastError();  // clear the sticky per-thread error of a failed launch
   return false;
 }
+
+// ------------------------------------------------------------ roctx ranges ---
+// Every public entry point opens a roctx range named after it, so a marker trace
+// (rocprofv3 --marker-trace) shows the library's calls beside their kernels.  Without a tool
+// attached the calls do nothing.
+namespace {
+struct Range {
+  explicit Range(const char* name) { roctxRangePushA(name); }
+  ~Range() { roctxRangePop(); }
+  Range(const Range&) = delete;
+  Range& operator=(const Range&) = delete;
+};
+}  // namespace
 
 // ---------------------------------------------------- device allocation ---
 // Every device allocation of the library goes through device_malloc, which also serves the
@@ -1000,14 +1014,17 @@ static void register_groups(VisConfig* c, MeshlessDataset* ds, bool async, const
 }
 
 void vis_register_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+  const Range range("vis_register_meshless_dataset");
   register_groups(c, ds, false, "vis_register_meshless_dataset");
 }
 
 void vis_register_meshless_dataset_async(VisConfig* c, MeshlessDataset* ds) {
+  const Range range("vis_register_meshless_dataset_async");
   register_groups(c, ds, true, "vis_register_meshless_dataset_async");
 }
 
 void vis_unregister_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
+  const Range range("vis_unregister_meshless_dataset");
   clear_error();
   if (!ds) return;
   if (c && c->_plan) (void)hipStreamSynchronize(c->_plan->stream);
@@ -1025,6 +1042,7 @@ void vis_unregister_meshless_dataset(VisConfig* c, MeshlessDataset* ds) {
 }
 
 void fourier_transform(MeshlessDataset* ds, VisConfig* c) {
+  const Range range("fourier_transform");
   clear_error();
   if (!ds || !c) {
     set_error(VIS_ERROR_INVALID_VALUE, "fourier_transform: null argument");
@@ -1041,6 +1059,7 @@ void fourier_transform(MeshlessDataset* ds, VisConfig* c) {
 }
 
 void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
+  const Range range("vis_fourier_volume_rendering");
   clear_error();
   if (!ds || !c) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering: null argument");
@@ -1075,6 +1094,7 @@ void vis_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c) {
 }
 
 void vis_fourier_volume_rendering_noalloc(MeshlessDataset* ds, VisConfig* c) {
+  const Range range("vis_fourier_volume_rendering_noalloc");
   clear_error();
   if (!ds || !c) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_noalloc: null argument");
@@ -1106,6 +1126,7 @@ void vis_fourier_volume_rendering_noalloc(MeshlessDataset* ds, VisConfig* c) {
 // Nx * Ny floats.  A failure of the interop calls (no current GL context, a name that is not a
 // buffer object) is reported as VIS_ERROR_HIP and renders nothing.
 void vis_opengl_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c, unsigned int buffer_object) {
+  const Range range("vis_opengl_fourier_volume_rendering");
   clear_error();
   if (!ds || !c) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_opengl_fourier_volume_rendering: null argument");
@@ -1148,6 +1169,7 @@ void vis_opengl_fourier_volume_rendering(MeshlessDataset* ds, VisConfig* c, unsi
 }
 
 void vis_copy_to_host(VisConfig* c, float* h_image) {
+  const Range range("vis_copy_to_host");
   clear_error();
   if (!c || !h_image || !c->_d_image) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_copy_to_host: null argument or no image");
@@ -1161,6 +1183,7 @@ void vis_copy_to_host(VisConfig* c, float* h_image) {
 }
 
 void vis_copy_spectrum_to_host(VisConfig* c, float2* h_spectrum) {
+  const Range range("vis_copy_spectrum_to_host");
   clear_error();
   if (!c || !h_spectrum) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_copy_spectrum_to_host: null argument");
@@ -1188,6 +1211,7 @@ void vis_synchronize(VisConfig* c) {
 }
 
 void vis_fourier_volume_rendering_batched(MeshlessDataset** datasets, VisConfig** configs, int n) {
+  const Range range("vis_fourier_volume_rendering_batched");
   clear_error();
   if (n < 0 || (n > 0 && (!datasets || !configs))) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_fourier_volume_rendering_batched: bad arguments");
@@ -1211,6 +1235,7 @@ void vis_fourier_volume_rendering_batched(MeshlessDataset** datasets, VisConfig*
 }
 
 double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
+  const Range range("vis_autotune");
   clear_error();
   if (!ds || !c || !(tolerance > 0.0)) {
     set_error(VIS_ERROR_INVALID_VALUE, "vis_autotune: null argument or non-positive tolerance");
