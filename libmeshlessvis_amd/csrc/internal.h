@@ -116,19 +116,14 @@ hipError_t launch_prep_packets(const Constraint* d_constraints, const float* d_r
                                double sx, double sy, double3 u, double3 v, float cull, Packet* out, hipStream_t s,
                                int* zero = nullptr, int nzero = 0);
 
-hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
-                                      int n_slots, double sx, double sy, double3 u, double3 v, float cull,
-                                      Packet* out, hipStream_t s, int* zero = nullptr, int nzero = 0);
-
 // The cull factor of a group under a config: 0 unless cull_fully_aliased_terms is set;
 // else the basis's support in units of r0 (SPH 2, Wendland 1, Gaussian 3: exp(-9 pi)).
 float cull_factor(const VisConfig* c, int basis);
 
 // One job of a batched packet preparation: the packets of a (view, group) pair.
 struct PrepJob {
-  const Constraint* c;
+  const Constraint* c;  // slot k holds particle k (k < n), zeros past n
   const float* radii;   // nullptr: no radii
-  const int* slot_map;  // nullptr: slot k is particle k (k < n), else slot_map[k] (negative: padding)
   Packet* out;
   int n, n_slots;
   double sx, sy;

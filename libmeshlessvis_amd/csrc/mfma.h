@@ -70,8 +70,10 @@ bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt
 // prepare + enqueue.
 bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
 
-// Slot -> particle map of a planned job (nullptr: slot i holds particle i, zeros past n).
-const int* mfma_slot_map(const MfmaPlanned& pl);
+// A planned job's particles and radii in slot order (nullptr: slot i holds particle i of the
+// group, zeros past n).
+const Constraint* mfma_ordered(const MfmaPlanned& pl);
+const float* mfma_ordered_radii(const MfmaPlanned& pl);
 
 // Which kernel variant a group takes (MfmaKind).
 int mfma_kind(const GroupInfo& info, int basis);

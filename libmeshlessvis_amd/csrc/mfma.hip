@@ -708,7 +708,8 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   return true;
 }
 
-const int* mfma_slot_map(const MfmaPlanned& pl) { return pl.bc ? pl.bc->slot_map.as<int>() : nullptr; }
+const Constraint* mfma_ordered(const MfmaPlanned& pl) { return pl.bc ? pl.bc->ordered.as<Constraint>() : nullptr; }
+const float* mfma_ordered_radii(const MfmaPlanned& pl) { return pl.bc ? pl.bc->ordered_r.as<float>() : nullptr; }
 
 // Device half: kernel launches only (safe to capture into a HIP graph).
 bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev) {
@@ -723,11 +724,11 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
     int* zero = k == 0 && fused ? bt.counters : nullptr;
     const int nzero = k == 0 && fused ? nkeys : 0;
     hipError_t e;
-    if (pl.bc)
-      e = launch_prep_packets_mapped(sp.g->d_constraints, sp.g->d_radii, pl.bc->slot_map.as<int>(), pl.n_slots,
-                                     sp.c->step_size.x, sp.c->step_size.y, sp.u, sp.v,
-                                     cull_factor(sp.c, sp.g->basis_function_id), const_cast<Packet*>(pl.job.packets), s,
-                                     zero, nzero);
+    if (pl.bc)  // the binned particles in slot order: every slot read in place
+      e = launch_prep_packets(pl.bc->ordered.as<Constraint>(), pl.bc->ordered_r.as<float>(), pl.n_slots, pl.n_slots,
+                              sp.c->step_size.x, sp.c->step_size.y, sp.u, sp.v,
+                              cull_factor(sp.c, sp.g->basis_function_id), const_cast<Packet*>(pl.job.packets), s,
+                              zero, nzero);
     else
       e = launch_prep_packets(sp.g->d_constraints, sp.g->d_radii, sp.n, pl.n_slots, sp.c->step_size.x,
                               sp.c->step_size.y, sp.u, sp.v, cull_factor(sp.c, sp.g->basis_function_id),

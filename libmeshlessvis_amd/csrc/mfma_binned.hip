@@ -644,8 +644,7 @@ static bool build_bins_host(const float* h, const Constraint* cons, const GroupI
   std::vector<int> start(nb + 1, 0);
   for (int b = 0; b < nb; ++b) start[b + 1] = start[b] + ((count[b] + kBinPad - 1) / kBinPad) * kBinPad;
   const int n_slots = std::max(start[nb], kBinPad);
-  std::vector<int>& map = bc->h_map;
-  map.assign(n_slots, -1);
+  std::vector<int> map(n_slots, -1);
   std::vector<int> fill(start.begin(), start.end() - 1);
   for (int i = 0; i < n; ++i) map[fill[bin[i]]++] = i;
   const int steps = n_slots / kStep;
@@ -699,14 +698,26 @@ static bool build_bins_host(const float* h, const Constraint* cons, const GroupI
   bc->n_slots = n_slots;
   bc->bin_step.resize(nb + 1);
   for (int b = 0; b <= nb; ++b) bc->bin_step[b] = start[b] / kStep;
-  if (!bc->slot_map.reserve(sizeof(int) * n_slots) || !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
+  // The particles in slot order (zero on padding slots): the packet prep reads them contiguously.
+  std::vector<Constraint>& ord = bc->h_ordered;
+  std::vector<float>& ord_r = bc->h_ordered_r;
+  ord.assign(n_slots, Constraint{});
+  ord_r.assign(n_slots, 0.0f);
+  for (int k = 0; k < n_slots; ++k)
+    if (map[k] >= 0) {
+      ord[k] = cons[map[k]];
+      ord_r[k] = h[map[k]];
+    }
+  if (!bc->ordered.reserve(sizeof(Constraint) * n_slots) || !bc->ordered_r.reserve(sizeof(float) * n_slots) ||
+      !bc->step_bin.reserve(sizeof(uint16_t) * steps) ||
       !bc->nodes.reserve(sizeof(float) * nodes.size()) || !bc->lag.reserve(sizeof(float) * lag.size()) ||
       !bc->bininfo.reserve(sizeof(float4) * nb))
     return false;
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hip_ok(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s), "upload bins");
   };
-  return up(bc->slot_map.ptr, map.data(), sizeof(int) * n_slots) &&
+  return up(bc->ordered.ptr, ord.data(), sizeof(Constraint) * n_slots) &&
+         up(bc->ordered_r.ptr, ord_r.data(), sizeof(float) * n_slots) &&
          up(bc->step_bin.ptr, sbin.data(), sizeof(uint16_t) * steps) &&
          up(bc->nodes.ptr, nodes.data(), sizeof(float) * nodes.size()) &&
          up(bc->lag.ptr, lag.data(), sizeof(float) * lag.size()) &&
@@ -735,7 +746,7 @@ static const BinCache* bin_cache_impl(const Group& g, const GroupInfo& info, int
   std::lock_guard<std::mutex> lk(g_bin_mu);
   auto key = std::make_tuple(static_cast<const void*>(g.d_constraints), nb, T);
   BinCache& e = bin_caches()[key];
-  if (e.n != n || !e.slot_map.ptr) {
+  if (e.n != n || !e.ordered.ptr) {
     e.release();
     const bool ok = from_host ? build_bins_host(g.h_radii, g.h_constraints, info, n, nb, T, s, &e)
                               : build_bins(g, info, n, nb, T, s, &e);

@@ -195,33 +195,38 @@ __device__ __forceinline__ u32x4_t swap_halves(u32x4_t v) {  // (r, i) -> (i, r)
   return (v >> 16) | (v << 16);
 }
 
-// h-binning of a group (mfma_binned.hip): slot k of the per-view packet array
-// holds particle slot_map[k] (-1: zero padding); every bin starts on a
-// 16-particle step; step_bin gives each step's bin.
+// h-binning of a group (mfma_binned.hip): slot k of the per-view packet array holds one
+// particle of the group (the particles sorted by bin, each bin padded with zero slots to whole
+// 16-particle steps); step_bin gives each step's bin.  The particles are kept in that slot
+// order (ordered, ordered_r), so the per-view packet prep reads them contiguously.
 // float4 words per slot holding the T Lagrange weights of a particle.
 constexpr int lag_words(int T) { return (T + 3) / 4; }
 
 struct BinCache {
   int n = 0, nb = 0, T = 0;
   int n_slots = 0;
-  DevBuf slot_map;  // int[n_slots]
-  DevBuf step_bin;  // uint16_t[n_slots / 16]
+  DevBuf ordered;    // Constraint[n_slots]: the particle of slot k, zero on padding
+  DevBuf ordered_r;  // float[n_slots]: its radius, zero on padding
+  DevBuf step_bin;   // uint16_t[n_slots / 16]
   DevBuf nodes;     // float[nb][kMaxNodes] Chebyshev nodes in h
   DevBuf lag;       // float4[n_slots][lag_words(T)]: Lagrange weights l_t(h_i) / max |w| of the bin, 0 on padding
   DevBuf bininfo;   // float4[nb] (h_lo^2, h_hi^2, max |w|, 1 / max |w|)
   std::vector<int> bin_step;  // host: first step of every bin, nb + 1 entries
   // Host copies of the tables: the sources of their stream-ordered uploads, kept until release.
-  std::vector<int> h_map;
+  std::vector<Constraint> h_ordered;
+  std::vector<float> h_ordered_r;
   std::vector<uint16_t> h_sbin;
   std::vector<float> h_nodes, h_lag;
   std::vector<float4> h_binfo;
   void release() {
-    slot_map.release();  // hipFree waits for the device, so the uploads above are done
+    ordered.release();  // hipFree waits for the device, so the uploads above are done
+    ordered_r.release();
     step_bin.release();
     nodes.release();
     lag.release();
     bininfo.release();
-    h_map.clear();
+    h_ordered.clear();
+    h_ordered_r.clear();
     h_sbin.clear();
     h_nodes.clear();
     h_lag.clear();

@@ -66,20 +66,8 @@ __global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __r
   out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v, cull);
 }
 
-// Same packets in a given slot order: slot k takes particle slot_map[k] (zero packet when negative).
-__global__ __launch_bounds__(256) void prep_packets_mapped_kernel(const Constraint* __restrict__ c,
-                                                                  const float* __restrict__ radii,
-                                                                  const int* __restrict__ slot_map, int n_slots,
-                                                                  double sx, double sy, double3 u, double3 v,
-                                                                  float cull, Packet* __restrict__ out, int* zero,
-                                                                  int nzero) {
-  clear_counters(zero, nzero);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n_slots) return;
-  out[k] = make_packet(c, radii, slot_map[k], sx, sy, u, v, cull);
-}
-
-// All jobs of a batch: thread g of the flattened slot range finds its job by binary search.
+// Every job's packets in one launch: a thread finds its job by binary search over the jobs'
+// first slots.
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restrict__ jobs,
                                                         const long long* __restrict__ begin, int n_jobs,
                                                         long long total, int* zero, int nzero) {
@@ -97,7 +85,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restric
   const PrepJob& J = jobs[lo];
   const int k = static_cast<int>(g - begin[lo]);
   if (k >= J.n_slots) return;
-  const int i = J.slot_map ? J.slot_map[k] : (k < J.n ? k : -1);
+  const int i = k < J.n ? k : -1;
   J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v, J.cull);
 }
 
@@ -113,17 +101,6 @@ hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_j
   const unsigned grid = prep_grid(std::max(0LL, total), nzero);
   if (grid == 0) return hipSuccess;
   hipLaunchKernelGGL(prep_jobs_kernel, dim3(grid), dim3(256), 0, s, jobs, begin, n_jobs, total, zero, nzero);
-  return hipGetLastError();
-}
-
-hipError_t launch_prep_packets_mapped(const Constraint* d_constraints, const float* d_radii, const int* slot_map,
-                                      int n_slots, double sx, double sy, double3 u, double3 v, float cull,
-                                      Packet* out, hipStream_t s, int* zero, int nzero) {
-  if (nzero > 256) return hipErrorInvalidValue;
-  const unsigned grid = prep_grid(std::max(0, n_slots), nzero);
-  if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_packets_mapped_kernel, dim3(grid), dim3(256), 0, s, d_constraints, d_radii, slot_map,
-                     n_slots, sx, sy, u, v, cull, out, zero, nzero);
   return hipGetLastError();
 }
 

@@ -527,11 +527,12 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
     const MfmaJobSpec& sp = specs[k];
     const MfmaPlanned& pl = bt.plans[k];
     PrepJob& J = pj[k];
-    J.c = sp.g->d_constraints;
-    J.radii = sp.g->d_radii;
-    J.slot_map = mfma_slot_map(pl);
+    // Binned jobs read their particles in slot order (every slot in place, zero on padding).
+    const Constraint* ord = mfma_ordered(pl);
+    J.c = ord ? ord : sp.g->d_constraints;
+    J.radii = ord ? mfma_ordered_radii(pl) : sp.g->d_radii;
     J.out = const_cast<Packet*>(pl.job.packets);
-    J.n = sp.n;
+    J.n = ord ? pl.n_slots : sp.n;
     J.n_slots = pl.n_slots;
     J.sx = sp.c->step_size.x;
     J.sy = sp.c->step_size.y;
