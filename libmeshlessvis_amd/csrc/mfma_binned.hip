@@ -247,9 +247,20 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 // in one bin); the other waves multiply the current one: consumer c = wave - 4 holds node
 // t = c / NB of the 32 x 32 block c % NB (NB = TX TY / 1024 blocks) - 4 products of
 // accumulators and the folded S(+-x) sums.  With the default tile, waves w, w + 4 and w + 8
-// share a SIMD: one producer and both nodes' consumers of block w.  An item owns chunk slots 2 * chunk and 2 * chunk + 1 (the
-// reduction adds them): node t's consumers write its folded sums into slot
-// 2 * chunk + t and its producers the x = 0 column.
+// share a SIMD: one producer and both nodes' consumers of block w.  An item owns chunk
+// slots 2 * chunk and 2 * chunk + 1 (the reduction adds them): node t's consumers write its
+// folded sums into slot 2 * chunk + t and its producers the x = 0 column.
+//
+// Split FP16: the two correction products of each node (A_lo B_t,hi + A_hi B_t,lo) are
+// summed for both nodes under one node's factor.  With E = w e^(2 pi i y b) / W_b (node
+// free: l_0 + l_1 = 1) and D = E - B_0,hi - B_1,hi (the nodes' low parts together),
+//     node 0 accumulates A_hi B_0,hi + A_lo E_hi,  node 1  A_hi B_1,hi + A_hi rnd(D),
+// E_hi = rnd(B_0,hi + B_1,hi),
+// so each consumer issues 8 MFMAs per 16 particles instead of 12.  What the node 1 term
+// of A_lo B_1 and the node 0 term of A_hi B_0,lo lose is
+// (Phi(r h_1) - Phi(r h_0)) / Phi(0) <= ~1% (ratio-1.01 bins) of a 2^-11-relative
+// correction: about 1e-6 relative.  Split BF16 (low parts 2^-8) keeps the three
+// products of each node.
 constexpr int kPairStage = 2 * kStep;             // particles per stage
 constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
 // 4 producer waves and two consumer waves (nodes 0, 1) per 32 x 32 block of the tile.
@@ -260,6 +271,7 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
     const MfmaJob* __restrict__ jobs, const MfmaItem* __restrict__ items, MfmaShared sh, int* __restrict__ counter) {
   static_assert(TX % 32 == 0 && TY % 32 == 0, "tile");
   constexpr int T = 2;
+  constexpr bool kShared = SPLIT && OP == OP_F16;  // shared correction products (above)
   constexpr int NB = (TX / 32) * (TY / 32);  // 32 x 32 blocks of the tile
   constexpr int kA = TX * kPairPitch;  // words of one A part
   constexpr int kB = TY * kPairPitch;  // words of one B part
@@ -376,11 +388,12 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
                                     im);
         }
         // Node weights w * l_t once per particle, then one multiply per value and node.
-        float wl[2][T];
+        float wl[2][T], ws[2];
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           wl[u][0] = a.q[u].w * a.l[u].x;
           wl[u][1] = a.q[u].w * a.l[u].y;
+          ws[u] = wl[u][0] + wl[u][1];  // w / W_b: the weights of E
         }
 #pragma unroll
         for (int m = 0; m < MB; ++m) {  // B_t(i, y)
@@ -391,6 +404,40 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
             PB[u] += dB[u];
             c0[u] = cos_rev(ph);
             s0[u] = sin_rev(ph);
+          }
+          const int o = (gg + 16 * m) * kPairPitch + pp;
+          if (kShared) {
+            // Parts 0, 2 of node t: B_t,hi; parts 1, 3: E_hi for node 0, rnd(D) for node 1.
+            uint32_t hre[T], him[T];
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+              float re[2], im[2];
+#pragma unroll
+              for (int u = 0; u < 2; ++u) {
+                re[u] = wl[u][t] * c0[u];
+                im[u] = wl[u][t] * s0[u];
+              }
+              if (decltype(zero)::value) {
+                zr[t][m] += re[0] + re[1];
+                zi[t][m] += im[0] + im[1];
+              }
+              hre[t] = pack2<OP>(re[0], re[1]);
+              him[t] = pack2<OP>(im[0], im[1]);
+              ldsB(buf, t, 0)[o] = hre[t];
+              ldsB(buf, t, 2)[o] = him[t];
+            }
+            // rnd(B_0,hi + B_1,hi) for rnd(E): within 2^-11 of it, an error of 2^-22 in
+            // A_lo rnd(E).  D = E - B_0,hi - B_1,hi from the FP32 E = (w / W_b) e^(...) by two
+            // FMAs per value, both exact (E is B_0 + B_1 up to FP32 rounding).
+            ldsB(buf, 0, 1)[o] = add_f16x2(hre[0], hre[1]);
+            ldsB(buf, 0, 3)[o] = add_f16x2(him[0], him[1]);
+            const float dr0 = minus_half<0>(fma_minus_half<0>(ws[0], c0[0], hre[0]), hre[1]);
+            const float dr1 = minus_half<1>(fma_minus_half<1>(ws[1], c0[1], hre[0]), hre[1]);
+            const float di0 = minus_half<0>(fma_minus_half<0>(ws[0], s0[0], him[0]), him[1]);
+            const float di1 = minus_half<1>(fma_minus_half<1>(ws[1], s0[1], him[0]), him[1]);
+            ldsB(buf, 1, 1)[o] = pack2<OP>(dr0, dr1);
+            ldsB(buf, 1, 3)[o] = pack2<OP>(di0, di1);
+            continue;
           }
 #pragma unroll
           for (int t = 0; t < T; ++t) {
@@ -404,8 +451,7 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
               zr[t][m] += re[0] + re[1];
               zi[t][m] += im[0] + im[1];
             }
-            store_pair<OP, SPLIT, kB>(reinterpret_cast<uint32_t(*)[kB]>(ldsB(buf, t, 0)),
-                                      (gg + 16 * m) * kPairPitch + pp, re, im);
+            store_pair<OP, SPLIT, kB>(reinterpret_cast<uint32_t(*)[kB]>(ldsB(buf, t, 0)), o, re, im);
           }
         }
       };
@@ -475,6 +521,7 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
     // waves' MFMAs and folds.
     const int c = wave - kProducerWaves;  // 0 .. 2 NB - 1
     const int blk = c % NB, t = c / NB;
+    const int xp = t == 0 ? 1 : 0;  // shared corrections: A part of the second product (lo, hi)
     const int wx = blk % (TX / 32);
     const int wy = blk / (TX / 32);
     f32x16_t acc[4], sp[2], sm[2];  // [Prr, Pii, Pri, Pir] of node t; folded S(+x), S(-x)
@@ -535,7 +582,10 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
       for (int kk = 0; kk < kPairStage / 16; ++kk) {
         const u32x4_t ahr = lda(kk, 0), ahi = lda(kk, 2), bhr = ldb(kk, 0), bhi = ldb(kk, 2);
         quart(ahr, ahi, bhr, bhi);  // hi.hi
-        if (SPLIT) {
+        if (kShared) {  // node 0: A_lo E_hi; node 1: A_hi rnd(D)
+          const u32x4_t xr = lda(kk, xp), xi = lda(kk, xp + 2), yr = ldb(kk, 1), yi = ldb(kk, 3);
+          quart(xr, xi, yr, yi);
+        } else if (SPLIT) {
           const u32x4_t alr = lda(kk, 1), ali = lda(kk, 3);
           quart(alr, ali, bhr, bhi);  // lo.hi
           const u32x4_t blr = ldb(kk, 1), bli = ldb(kk, 3);

@@ -155,6 +155,35 @@ __device__ __forceinline__ uint32_t split_lo_f16(float a, float b, uint32_t hi) 
   return d;
 }
 
+// a * b - h rounded once (one v_fma_mix_f32), h the FP16 half U of the word hi taken as
+// FP32.
+template <int U>
+__device__ __forceinline__ float fma_minus_half(float a, float b, uint32_t hi) {
+  float d;
+  if (U == 0)
+    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(hi));
+  else
+    asm("v_fma_mix_f32 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(b), "v"(hi));
+  return d;
+}
+
+// a - h, h the FP16 half U of hi (one v_fma_mix_f32).
+template <int U>
+__device__ __forceinline__ float minus_half(float a, uint32_t hi) {
+  float d;
+  if (U == 0)
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(hi));
+  else
+    asm("v_fma_mix_f32 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "=v"(d) : "v"(a), "v"(hi));
+  return d;
+}
+
+// The FP16 pair sum x + y of two packed words (v_pk_add_f16, round to nearest even).
+__device__ __forceinline__ uint32_t add_f16x2(uint32_t x, uint32_t y) {
+  using h2 = _Float16 __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(h2, x) + __builtin_bit_cast(h2, y));
+}
+
 // rnd(a - hi.x), rnd(b - hi.y): the low words of the split a = hi + lo.
 template <int OP>
 __device__ __forceinline__ uint32_t split_lo(float a, float b, uint32_t hi) {

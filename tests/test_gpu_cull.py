@@ -6,7 +6,10 @@ particles placed well inside the window, well outside it, and straddling its edg
   * mixed data: the culled render equals the render of the data without the outside
     particles (relative L2 <= 1e-6: the sums run in another order) and the FP64 oracle
     of that data (<= 1e-5); straddling particles are kept;
-  * the batched graph is keyed on the flag (toggling it re-captures)."""
+  * the batched graph is keyed on the flag (toggling it re-captures).  Its culled renders
+    are held to <= 2e-6 of the kept-only render: the two bin different particle sets
+    (the outside particles keep their h-bin slots with weight 0), so their h-interpolation
+    errors, each about 1-2e-6 against FP64, differ; and to <= 1e-5 of the FP64 oracle."""
 import numpy as np
 import pytest
 
@@ -110,8 +113,11 @@ def test_cull_batched_graph_follows_the_flag():
             c.close()
     F_kept, _ = render(kept, "mfma_split_f16", "analytic", False)
     F_all, _ = render(groups, "mfma_split_f16", "analytic", False)
+    F64 = oracle.spectrum64(kept, CUTOFF, STEP, U, V, "analytic")
     for F in out[True][0] + out[True][1]:
-        assert oracle.rel_l2(F, F_kept) <= 1e-6
+        print(f"culled batch vs kept-only render {oracle.rel_l2(F, F_kept):.2e}, vs FP64 {oracle.rel_l2(F, F64):.2e}")
+        assert oracle.rel_l2(F, F_kept) <= 2e-6
+        assert oracle.rel_l2(F, F64) <= 1e-5
     for F in out[False][0]:
         assert oracle.rel_l2(F, F_all) <= 1e-6
     np.testing.assert_array_equal(out[True][0][0], out[True][1][0])
