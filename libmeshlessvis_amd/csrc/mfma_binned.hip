@@ -253,14 +253,14 @@ __global__ __launch_bounds__(kThreads, 1) void mfma_binned_kernel(const MfmaJob*
 //
 // Split FP16: the two correction products of each node (A_lo B_t,hi + A_hi B_t,lo) are
 // summed for both nodes under one node's factor.  With E = w e^(2 pi i y b) / W_b (node
-// free: l_0 + l_1 = 1) and D = E - B_0,hi - B_1,hi (the nodes' low parts together),
+// free: l_0 + l_1 = 1), B_1,hi = rnd(E - B_0,hi) and D = E - B_0,hi - B_1,hi (the nodes'
+// low parts together),
 //     node 0 accumulates A_hi B_0,hi + A_lo E_hi,  node 1  A_hi B_1,hi + A_hi rnd(D),
-// E_hi = rnd(B_0,hi + B_1,hi),
-// so each consumer issues 8 MFMAs per 16 particles instead of 12.  What the node 1 term
-// of A_lo B_1 and the node 0 term of A_hi B_0,lo lose is
-// (Phi(r h_1) - Phi(r h_0)) / Phi(0) <= ~1% (ratio-1.01 bins) of a 2^-11-relative
-// correction: about 1e-6 relative.  Split BF16 (low parts 2^-8) keeps the three
-// products of each node.
+// E_hi = rnd(B_0,hi + B_1,hi): with equal factors this is A.E to the split's precision, and
+// each consumer issues 8 MFMAs per 16 particles instead of 12.  A_lo B_1 goes with node 0's
+// factor and A_hi B_0,lo with node 1's: an error of (Phi(r h_1) - Phi(r h_0)) / Phi(0)
+// (<= ~1% for ratio-1.01 bins) times a 2^-11-relative term, about 1e-6 relative.  Split
+// BF16 (low parts 2^-8) keeps the three products of each node.
 constexpr int kPairStage = 2 * kStep;             // particles per stage
 constexpr int kPairPitch = kPairStage / 2 + 4;    // 16 words of 32 particles + 4 against bank conflicts
 // 4 producer waves and two consumer waves (nodes 0, 1) per 32 x 32 block of the tile.
@@ -408,35 +408,35 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
           const int o = (gg + 16 * m) * kPairPitch + pp;
           if (kShared) {
             // Parts 0, 2 of node t: B_t,hi; parts 1, 3: E_hi for node 0, rnd(D) for node 1.
-            uint32_t hre[T], him[T];
+            // Node 0: B_0 in FP32 and its high part.  Node 1 from v = E - B_0,hi (one FMA per
+            // value, from the FP32 E = (w / W_b) e^(...)): B_1,hi = rnd(v), D = v - B_1,hi
+            // (exact), so B_0,hi + B_1,hi + D = E; v is B_1 plus B_0's low part, which node 1
+            // thus carries - under its own factor, like the shared corrections.
+            float r0[2], i0[2];
 #pragma unroll
-            for (int t = 0; t < T; ++t) {
-              float re[2], im[2];
-#pragma unroll
-              for (int u = 0; u < 2; ++u) {
-                re[u] = wl[u][t] * c0[u];
-                im[u] = wl[u][t] * s0[u];
-              }
-              if (decltype(zero)::value) {
-                zr[t][m] += re[0] + re[1];
-                zi[t][m] += im[0] + im[1];
-              }
-              hre[t] = pack2<OP>(re[0], re[1]);
-              him[t] = pack2<OP>(im[0], im[1]);
-              ldsB(buf, t, 0)[o] = hre[t];
-              ldsB(buf, t, 2)[o] = him[t];
+            for (int u = 0; u < 2; ++u) {
+              r0[u] = wl[u][0] * c0[u];
+              i0[u] = wl[u][0] * s0[u];
             }
-            // rnd(B_0,hi + B_1,hi) for rnd(E): within 2^-11 of it, an error of 2^-22 in
-            // A_lo rnd(E).  D = E - B_0,hi - B_1,hi from the FP32 E = (w / W_b) e^(...) by two
-            // FMAs per value, both exact (E is B_0 + B_1 up to FP32 rounding).
-            ldsB(buf, 0, 1)[o] = add_f16x2(hre[0], hre[1]);
-            ldsB(buf, 0, 3)[o] = add_f16x2(him[0], him[1]);
-            const float dr0 = minus_half<0>(fma_minus_half<0>(ws[0], c0[0], hre[0]), hre[1]);
-            const float dr1 = minus_half<1>(fma_minus_half<1>(ws[1], c0[1], hre[0]), hre[1]);
-            const float di0 = minus_half<0>(fma_minus_half<0>(ws[0], s0[0], him[0]), him[1]);
-            const float di1 = minus_half<1>(fma_minus_half<1>(ws[1], s0[1], him[0]), him[1]);
-            ldsB(buf, 1, 1)[o] = pack2<OP>(dr0, dr1);
-            ldsB(buf, 1, 3)[o] = pack2<OP>(di0, di1);
+            const uint32_t h0r = pack2<OP>(r0[0], r0[1]), h0i = pack2<OP>(i0[0], i0[1]);
+            const float vr0 = fma_minus_half<0>(ws[0], c0[0], h0r), vr1 = fma_minus_half<1>(ws[1], c0[1], h0r);
+            const float vi0 = fma_minus_half<0>(ws[0], s0[0], h0i), vi1 = fma_minus_half<1>(ws[1], s0[1], h0i);
+            const uint32_t h1r = pack2<OP>(vr0, vr1), h1i = pack2<OP>(vi0, vi1);
+            if (decltype(zero)::value) {  // x = 0: the exact FP32 values of each node
+              zr[0][m] += r0[0] + r0[1];
+              zi[0][m] += i0[0] + i0[1];
+              zr[1][m] += wl[0][1] * c0[0] + wl[1][1] * c0[1];
+              zi[1][m] += wl[0][1] * s0[0] + wl[1][1] * s0[1];
+            }
+            ldsB(buf, 0, 0)[o] = h0r;
+            ldsB(buf, 0, 2)[o] = h0i;
+            ldsB(buf, 1, 0)[o] = h1r;
+            ldsB(buf, 1, 2)[o] = h1i;
+            // rnd(B_0,hi + B_1,hi) for rnd(E): within 2^-11 of it, an error of 2^-22 in A_lo E_hi.
+            ldsB(buf, 0, 1)[o] = add_f16x2(h0r, h1r);
+            ldsB(buf, 0, 3)[o] = add_f16x2(h0i, h1i);
+            ldsB(buf, 1, 1)[o] = pack2<OP>(minus_half<0>(vr0, h1r), minus_half<1>(vr1, h1r));
+            ldsB(buf, 1, 3)[o] = pack2<OP>(minus_half<0>(vi0, h1i), minus_half<1>(vi1, h1i));
             continue;
           }
 #pragma unroll
