@@ -679,7 +679,9 @@ static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
     if (key != bg.seen) {
       if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
       const bool ok = enqueue_batch(ds, cf, n, &bg, false);
-      bg.seen = ok ? key : std::vector<uint64_t>();
+      // The key as the same batch will present it next time: the first render of a config
+      // allocates buffers the key names (the FP64 accumulator, the C2R workspace).
+      bg.seen = ok ? batch_key(ds, cf, n) : std::vector<uint64_t>();
       if (ok && ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
       return;
     }
@@ -687,7 +689,7 @@ static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
       bg.drop();
       return;
     }
-    bg.key = key;
+    bg.key = batch_key(ds, cf, n);
   }
   if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
   if (!hip_ok(hipGraphLaunch(bg.exec, p0->stream), "hipGraphLaunch")) return;
