@@ -397,6 +397,12 @@ static void launch_sum_kind(int grid, hipStream_t s, const MfmaJob* jobs, const 
   }
 }
 
+// Extra time per stage of a two-node tile that also sums the x = 0 column, relative to the
+// other tiles: measured, one render of the 262k clustered view in one round of items, their
+// items took 196 against 178 us for the same steps (their producers' stage loop issues 256
+// instead of 181 VALU instructions, but the producers are not all of an item's time).
+constexpr double kZeroColExtra = 0.10;
+
 static int device_cus() {
   static int cus = 0;
   if (!cus) {
@@ -511,6 +517,12 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
   const int cap = binned ? 4096 : kMaxChunkSteps;
   const int floor_steps = binned ? 128 : 32;
   out->tile_steps = total_steps * tiles;
+  // One render of the two-node kernel: the tiles of the first |x| range, whose producers also
+  // sum the x = 0 column (kZeroColExtra more work per stage), get proportionally more and
+  // shorter step ranges than the other tiles, so that one round of items ends together.
+  const int zero_tiles = tiles / job.tiles_x;  // tiles t with t % tiles_x == 0
+  const bool split_zero = target_steps <= 0 && paired && job.tiles_x > 1 && spec.vp.cx > 0;
+  long long item_budget = 0;  // one render: the items of its rounds (rounds x CUs)
   if (target_steps <= 0) {
     // One render: items of equal size whose count is a whole number (1 or 2) of rounds over
     // the CUs - the atomic queue hands equal items out in rounds, and a last round of a few
@@ -526,12 +538,26 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
     const long long max_rounds = binned ? 1 : 2;
     const long long rounds =
         std::max(1LL, std::min(max_rounds, std::llround(w / (static_cast<double>(floor_steps) * cus))));
-    const long long chunks = std::max(1LL, (rounds * cus + tiles - 1) / tiles);
+    item_budget = rounds * cus;
+    const long long chunks =
+        split_zero ? std::max(1LL, std::min<long long>(std::llround(rounds * cus / (tiles + zero_tiles * kZeroColExtra)),
+                                                      rounds * cus / tiles))
+                   : std::max(1LL, (rounds * cus + tiles - 1) / tiles);
     target_steps = (total_steps + chunks - 1) / chunks;
   }
   int chunk_steps = static_cast<int>(std::min<long long>(cap, std::max<long long>(floor_steps, target_steps)));
   chunk_steps += chunk_steps & 1;  // even: items of the two-node kernel are whole 32-particle stages
   int chunk = 0;
+  auto add_item = [&](int t, int bin, int s0, int s1, int cost, int ck) {
+    MfmaItem it{};
+    it.tile = t;
+    it.s0 = s0;
+    it.s1 = s1;
+    it.bin = bin;
+    it.chunk = ck;
+    it.cost = cost;
+    out->items.push_back(it);
+  };
   auto add_range = [&](int bin, int s0, int s1, int cost) {
     for (int t = 0; t < tiles; ++t) {
       MfmaItem it{};
@@ -553,26 +579,56 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
     // kFoldCost more (the fold of the accumulators and of the x = 0 column at a bin change,
     // and the stage it holds up), so the ranges over the sparse tails of the h distribution,
     // one short bin after another, are not the stragglers of the launch.  The value is
-    // measured: on the 262k-particle clustered view 2, 3, 5, 8, 12, 20 give 0.378, 0.353,
-    // 0.313, 0.277, 0.276, 0.287 ms.
-    constexpr double kFoldCost = 8.0;
+    // measured: a least-squares fit of the item durations of one render of the 262k-particle
+    // clustered view gives 1.11 us per 32-particle stage and 2.74 us per fold, about 5 steps.
+    constexpr double kFoldCost = 5.0;
     std::vector<double> cum(static_cast<size_t>(total_steps) + 1, 0.0);
     std::vector<char> starts(static_cast<size_t>(total_steps) + 1, 0);
     for (size_t b = 0; b + 1 < bin_step.size(); ++b)
       if (bin_step[b] < bin_step[b + 1]) starts[bin_step[b]] = 1;
     for (long long k = 0; k < total_steps; ++k) cum[k + 1] = cum[k] + 1.0 + (starts[k] ? kFoldCost : 0.0);
-    const long long n_ranges = std::max<long long>(1, (total_steps + chunk_steps - 1) / chunk_steps);
-    long long s0 = 0;
-    for (long long r = 1; r <= n_ranges && s0 < total_steps; ++r) {
-      long long s1 = total_steps;
-      if (r < n_ranges) {
-        const double goal = cum[total_steps] * static_cast<double>(r) / static_cast<double>(n_ranges);
-        s1 = std::lower_bound(cum.begin(), cum.end(), goal) - cum.begin();
-        s1 += s1 & 1;  // whole 32-particle stages
-        s1 = std::min(total_steps, std::max(s1, s0 + 2));
+    // n ranges [s0, s1) of about equal cost, in whole 32-particle stages.
+    auto partition = [&](long long n) {
+      std::vector<std::pair<int, int>> r;
+      long long s0 = 0;
+      for (long long k = 1; k <= n && s0 < total_steps; ++k) {
+        long long s1 = total_steps;
+        if (k < n) {
+          const double goal = cum[total_steps] * static_cast<double>(k) / static_cast<double>(n);
+          s1 = std::lower_bound(cum.begin(), cum.end(), goal) - cum.begin();
+          s1 += s1 & 1;
+          s1 = std::min(total_steps, std::max(s1, s0 + 2));
+        }
+        r.emplace_back(static_cast<int>(s0), static_cast<int>(s1));
+        s0 = s1;
       }
-      add_range(0, static_cast<int>(s0), static_cast<int>(s1), static_cast<int>(cum[s1] - cum[s0] + 0.5));
-      s0 = s1;
+      return r;
+    };
+    const long long n_ranges = std::max<long long>(1, (total_steps + chunk_steps - 1) / chunk_steps);
+    auto range_cost = [&](const std::pair<int, int>& r) { return static_cast<int>(cum[r.second] - cum[r.first] + 0.5); };
+    if (split_zero) {
+      // Chunk k: range r0[k] of the x = 0 tiles, r1[k] of the others (an empty item, which
+      // writes zeros, where a tile has no range k: the reduction sums every chunk slot).
+      // As many items as the render's rounds hold: x = 0 tiles x n0 + others x n1 <= budget.
+      const long long others = tiles - zero_tiles;
+      long long n0 = std::llround(n_ranges * (1.0 + kZeroColExtra));
+      n0 = std::max<long long>(n_ranges, std::min<long long>(n0, (item_budget - others * n_ranges) / zero_tiles));
+      const auto r1 = partition(n_ranges);
+      const auto r0 = partition(n0);
+      const size_t n = std::max(r0.size(), r1.size());
+      for (size_t k = 0; k < n; ++k) {
+        for (int t = 0; t < tiles; ++t) {
+          const auto& r = t % job.tiles_x == 0 ? r0 : r1;
+          const double f = t % job.tiles_x == 0 ? 1.0 + kZeroColExtra : 1.0;
+          if (k < r.size())
+            add_item(t, 0, r[k].first, r[k].second, static_cast<int>(range_cost(r[k]) * f + 0.5), chunk);
+          else
+            add_item(t, 0, 0, 0, 0, chunk);
+        }
+        ++chunk;
+      }
+    } else {
+      for (const auto& r : partition(n_ranges)) add_range(0, r.first, r.second, range_cost(r));
     }
   } else {
     for (size_t b = 0; b + 1 < bin_step.size(); ++b)
