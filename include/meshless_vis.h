@@ -227,6 +227,13 @@ void vis_reset_kernel_timing(VisConfig* vis_config);
    the library's out-of-memory paths; process-wide. */
 void vis_debug_fail_device_allocations(int after);
 
+/* Test support (no counterpart in the reference): how many batch graphs the batched
+   renders led by this config (the first config of each batch) have captured, and how many
+   times they were launched, the capturing call included.  A batch's first call runs
+   without a graph, its second captures and launches the graph, later identical calls
+   replay it. */
+void vis_debug_batch_graph_counts(const VisConfig* vis_config, int* captures, long long* launches);
+
 /* Last error of the calling thread (VIS_SUCCESS / "" when none). */
 int vis_get_last_error_code(void);
 const char* vis_get_last_error(void);

@@ -150,6 +150,8 @@ struct BatchGraph {
   hipStream_t capture = nullptr;  // default priority
   mv::MfmaWorkspace ws;
   mv::DevBuf io;  // batched prep / finish / C2R tables
+  int captures = 0;        // graphs captured (over the config's life)
+  long long launches = 0;  // graph launches (the capturing call's included)
   void drop() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -690,9 +692,11 @@ static void render_batch_graph(MeshlessDataset** ds, VisConfig** cf, int n) {
       return;
     }
     bg.key = batch_key(ds, cf, n);
+    ++bg.captures;
   }
   if (ev && !hip_ok(hipEventRecord(ev->first, p0->stream), "hipEventRecord")) return;
   if (!hip_ok(hipGraphLaunch(bg.exec, p0->stream), "hipGraphLaunch")) return;
+  ++bg.launches;
   if (ev) hip_ok(hipEventRecord(ev->second, p0->stream), "hipEventRecord");
 }
 
@@ -1355,6 +1359,16 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
 }
 
 void vis_debug_fail_device_allocations(int after) { g_fail_after.store(after < 0 ? -1 : after); }
+
+void vis_debug_batch_graph_counts(const VisConfig* c, int* captures, long long* launches) {
+  clear_error();
+  if (!c || !c->_plan || !captures || !launches) {
+    set_error(VIS_ERROR_INVALID_VALUE, "vis_debug_batch_graph_counts: null argument");
+    return;
+  }
+  *captures = c->_plan->batch.captures;
+  *launches = c->_plan->batch.launches;
+}
 
 void vis_set_kernel_timing(VisConfig* c, int enable) {
   if (!c) return;

@@ -92,6 +92,8 @@ def test_failures_are_error_codes_not_aborts():
     assert L.vis_get_last_error_code() == 1
     L.vis_opengl_fourier_volume_rendering(None, None, 0)
     assert L.vis_get_last_error_code() == 1
+    L.vis_debug_batch_graph_counts(None, None, None)
+    assert L.vis_get_last_error_code() == 1
     L.vis_clear_last_error()
 
 

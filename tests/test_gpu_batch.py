@@ -144,3 +144,49 @@ def test_batched_graph_after_reregistration():
         cfgs[0].unregister(ds)
     for c in cfgs:
         c.close()
+
+
+def test_second_call_captures_third_replays():
+    """Configs whose first render is the batched call itself (their first render allocates
+    buffers the batch key names): the first call runs without a graph, the second captures
+    and launches it, the third replays it; a new view is captured anew."""
+    import ctypes as C
+    from libmeshlessvis_amd import abi
+
+    print("[case] batch graph counts", flush=True)
+    rng = np.random.default_rng(23)
+    sets = make_sets(rng)[:2]
+    vws = views(rng, 2)
+    pairs, cfgs = [], []
+    for ds in sets:
+        for vw in vws:
+            pairs.append(ds)
+            cfgs.append(host.Config(cutoff=(64, 32), samples=(128, 96), step=(0.5, 0.5), u=vw.u, v=vw.v,
+                                    form="mfma_split_f16", phi_mode="analytic"))
+    for ds in sets:
+        cfgs[0].register(ds)
+
+    def counts():
+        cap, launches = C.c_int(-1), C.c_longlong(-1)
+        abi.lib().vis_debug_batch_graph_counts(cfgs[0].p, C.byref(cap), C.byref(launches))
+        abi.check("vis_debug_batch_graph_counts")
+        return cap.value, launches.value
+
+    expected = [(0, 0), (1, 1), (1, 2), (1, 3)]
+    images = []
+    for call, want in enumerate(expected):
+        host.render_batched(pairs, cfgs)
+        images.append([cfg.image() for cfg in cfgs])
+        assert counts() == want, (call, counts())
+    for imgs in images[1:]:
+        for a, b in zip(imgs, images[0]):
+            assert np.array_equal(a, b)
+    cfgs[1].set_view(vws[0].u, vws[0].v)  # a new batch: its first call runs without a graph
+    host.render_batched(pairs, cfgs)
+    assert counts() == (1, 3)
+    host.render_batched(pairs, cfgs)
+    assert counts() == (2, 4)
+    for ds in sets:
+        cfgs[0].unregister(ds)
+    for c in cfgs:
+        c.close()
