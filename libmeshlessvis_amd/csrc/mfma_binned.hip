@@ -455,7 +455,7 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
           }
         }
       };
-      // Packets and Lagrange weights are loaded one stage ahead of their use, without a
+      // Packets and Lagrange weights are loaded two stages ahead of their use, without a
       // branch (a conditional load would make the compiler wait for it at the branch's
       // join, right after issuing it).  The x = 0 sums are folded when the stage about to
       // be generated opens a new bin, and every kBFlushSteps steps, as the consumers fold
@@ -463,14 +463,20 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
       auto produce = [&](auto zero) {
         int zb = 0, since = 0;
         if (n_stages > 0) {
-          Ahead a0, an;
+          // Three fixed register sets, stage k's packets in set k mod 3: the loads of stage
+          // k + 2 are issued as stage k is generated, two stages ahead of their use.  (With a
+          // set copied per stage the compiler sinks the loads into the middle of the stage, to
+          // reuse their registers, leaving half a stage to hide their latency.)
+          Ahead s0, s1, s2;
           if (decltype(zero)::value) zb = bin_at(0);
-          fetch(0, &a0);
-          fetch(min(1, n_stages - 1), &an);
-          generate(a0, 0, zero);
+          fetch(0, &s0);
+          fetch(min(1, n_stages - 1), &s1);
+          fetch(min(2, n_stages - 1), &s2);
+          generate(s0, 0, zero);
           since = 2;
           __syncthreads();
-          for (int k = 1; k < n_stages; ++k) {  // stage k while the consumers multiply k - 1
+          // Stage k (k >= 1) while the consumers multiply k - 1; refill the set of stage k - 1.
+          auto stage = [&](int k, const Ahead& use, Ahead* refill) {
             if (decltype(zero)::value) {
               const int nb = bin_at(k);
               if (nb != zb || since >= kBFlushSteps) {
@@ -479,11 +485,18 @@ __global__ __launch_bounds__(pair_threads(TX, TY), 1) void mfma_binned_pair_kern
                 since = 0;
               }
             }
-            const Ahead a = an;
-            fetch(min(k + 1, n_stages - 1), &an);
-            generate(a, k & 1, zero);
+            fetch(min(k + 2, n_stages - 1), refill);
+            __builtin_amdgcn_sched_barrier(0);  // the loads stay ahead of the generation
+            generate(use, k & 1, zero);
             since += 2;
             __syncthreads();
+          };
+          for (int k = 1; k < n_stages; k += 3) {
+            stage(k, s1, &s0);
+            if (k + 1 >= n_stages) break;
+            stage(k + 1, s2, &s1);
+            if (k + 2 >= n_stages) break;
+            stage(k + 2, s0, &s2);
           }
         }
         __syncthreads();
