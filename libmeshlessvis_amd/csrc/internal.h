@@ -131,10 +131,12 @@ struct PrepJob {
   float cull;  // as launch_prep_packets
   int pad;
 };
-// Every job's packets in one launch; begin[j] = first flattened slot of job j (prefix sums of
-// n_slots, ascending), total = all slots.
-hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s,
-                            int* zero = nullptr, int nzero = 0);
+// Every job's packets in one launch of kPrepBlock-thread blocks, each within one job:
+// block_job[b] = the job of block b, begin[j] = the first block of job j, n_blocks = all
+// blocks (the sum over jobs of ceil(n_slots / kPrepBlock)).
+constexpr int kPrepBlock = 256;
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, const int* block_job, long long n_blocks,
+                            hipStream_t s, int* zero = nullptr, int nzero = 0);
 
 // One render of a batch after the MFMA launches: its jobs' split-K partials
 // (JobPart[job_begin, job_end), summed in FP64 in a fixed order) give F = scale * (re, -im);

@@ -66,24 +66,18 @@ __global__ __launch_bounds__(256) void prep_packets_kernel(const Constraint* __r
   out[i] = make_packet(c, radii, i < n ? i : -1, sx, sy, u, v, cull);
 }
 
-// Every job's packets in one launch: a thread finds its job by binary search over the jobs'
-// first slots.
-__global__ __launch_bounds__(256) void prep_jobs_kernel(const PrepJob* __restrict__ jobs,
-                                                        const long long* __restrict__ begin, int n_jobs,
-                                                        long long total, int* zero, int nzero) {
+// Every job's packets in one launch: block b prepares slots [(b - begin[j]) * kPrepBlock, ...)
+// of job j = block_job[b] (a table lookup, uniform over the block).
+__global__ __launch_bounds__(kPrepBlock) void prep_jobs_kernel(const PrepJob* __restrict__ jobs,
+                                                               const long long* __restrict__ begin,
+                                                               const int* __restrict__ block_job, long long n_blocks,
+                                                               int* zero, int nzero) {
   clear_counters(zero, nzero);
-  const long long g = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= total) return;
-  int lo = 0, hi = n_jobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (begin[mid] <= g)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const PrepJob& J = jobs[lo];
-  const int k = static_cast<int>(g - begin[lo]);
+  const long long b = blockIdx.x;
+  if (b >= n_blocks) return;
+  const int j = block_job[b];
+  const PrepJob& J = jobs[j];
+  const int k = static_cast<int>((b - begin[j]) * kPrepBlock) + static_cast<int>(threadIdx.x);
   if (k >= J.n_slots) return;
   const int i = k < J.n ? k : -1;
   J.out[k] = make_packet(J.c, J.radii, i, J.sx, J.sy, J.u, J.v, J.cull);
@@ -94,13 +88,15 @@ static unsigned prep_grid(long long items, int nzero) {
   return static_cast<unsigned>(std::max<long long>(nzero > 0 ? 1 : 0, (items + 255) / 256));
 }
 
-hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, int n_jobs, long long total, hipStream_t s,
-                            int* zero, int nzero) {
+hipError_t launch_prep_jobs(const PrepJob* jobs, const long long* begin, const int* block_job, long long n_blocks,
+                            hipStream_t s, int* zero, int nzero) {
   if (nzero > 256) return hipErrorInvalidValue;
-  if (n_jobs <= 0) total = 0;
-  const unsigned grid = prep_grid(std::max(0LL, total), nzero);
+  n_blocks = std::max(0LL, n_blocks);
+  const long long grid = std::max<long long>(nzero > 0 ? 1 : 0, n_blocks);
   if (grid == 0) return hipSuccess;
-  hipLaunchKernelGGL(prep_jobs_kernel, dim3(grid), dim3(256), 0, s, jobs, begin, n_jobs, total, zero, nzero);
+  if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(prep_jobs_kernel, dim3(static_cast<unsigned>(grid)), dim3(kPrepBlock), 0, s, jobs, begin,
+                     block_job, n_blocks, zero, nzero);
   return hipGetLastError();
 }
 

@@ -517,11 +517,12 @@ static std::vector<uint64_t> batch_key(MeshlessDataset** ds, VisConfig** cf, int
 // Device tables of the batched prep / finish / C2R launches (uploaded before capture).
 static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, VisConfig** cf, int n,
                              const std::vector<int>& render_of, BatchGraph* bg, const PrepJob** jobs,
-                             const long long** begin, const RenderOut** outs, const JobPart** parts,
-                             long long* total) {
+                             const long long** begin, const int** block_job, const RenderOut** outs,
+                             const JobPart** parts, long long* total) {
   const size_t nj = specs.size();
   std::vector<PrepJob> pj(nj);
-  std::vector<long long> bg_begin(nj);
+  std::vector<long long> bg_begin(nj);  // first prep block of each job
+  std::vector<int> bj;                  // the job of each prep block
   std::vector<JobPart> jp(nj);
   std::vector<RenderOut> ro(n);
   long long acc = 0;
@@ -542,7 +543,9 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
     J.v = sp.v;
     J.cull = cull_factor(sp.c, sp.g->basis_function_id);
     bg_begin[k] = acc;
-    acc += pl.n_slots;
+    const long long blocks = (pl.n_slots + kPrepBlock - 1) / kPrepBlock;
+    acc += blocks;
+    bj.insert(bj.end(), static_cast<size_t>(blocks), static_cast<int>(k));
     jp[k].partial = pl.job.partial;
     jp[k].nchunks = pl.nchunks;
   }
@@ -553,9 +556,10 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
     while (ro[i].job_end < static_cast<int>(nj) && render_of[ro[i].job_end] == i) ++ro[i].job_end;
   }
   const size_t b_pj = sizeof(PrepJob) * nj, b_bg = sizeof(long long) * nj, b_ro = sizeof(RenderOut) * n,
-               b_jp = sizeof(JobPart) * nj;
+               b_jp = sizeof(JobPart) * nj, b_bj = sizeof(int) * bj.size();
   auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-  const size_t o_bg = up(b_pj), o_ro = o_bg + up(b_bg), o_jp = o_ro + up(b_ro), all = o_jp + up(b_jp);
+  const size_t o_bg = up(b_pj), o_ro = o_bg + up(b_bg), o_jp = o_ro + up(b_ro), o_bj = o_jp + up(b_jp),
+               all = o_bj + up(b_bj);
   // The tables of the previous batch (an eager run or a replay of its graph, queued on the
   // configs' stream, possibly a non-blocking one) may still be read: wait for them before the
   // buffer is reused.
@@ -566,12 +570,14 @@ static bool upload_io_tables(const std::vector<MfmaJobSpec>& specs, const MfmaBa
   std::memcpy(h.data() + o_bg, bg_begin.data(), b_bg);
   std::memcpy(h.data() + o_ro, ro.data(), b_ro);
   std::memcpy(h.data() + o_jp, jp.data(), b_jp);
+  if (b_bj) std::memcpy(h.data() + o_bj, bj.data(), b_bj);
   if (!hip_ok(hipMemcpy(bg->io.ptr, h.data(), all, hipMemcpyHostToDevice), "upload batch tables")) return false;
   char* base = static_cast<char*>(bg->io.ptr);
   *jobs = reinterpret_cast<const PrepJob*>(base);
   *begin = reinterpret_cast<const long long*>(base + o_bg);
   *outs = reinterpret_cast<const RenderOut*>(base + o_ro);
   *parts = reinterpret_cast<const JobPart*>(base + o_jp);
+  *block_job = reinterpret_cast<const int*>(base + o_bj);
   *total = acc;
   return true;
 }
@@ -607,11 +613,13 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
   if (!mfma_prepare(specs, s, &bg->ws, &bg->ws.batch)) return false;
   const PrepJob* pjobs = nullptr;
   const long long* pbegin = nullptr;
+  const int* pblock_job = nullptr;
   const RenderOut* outs = nullptr;
   const JobPart* parts = nullptr;
   long long total = 0;
   if (same_grid &&
-      !upload_io_tables(specs, bg->ws.batch, cf, n, render_of, bg, &pjobs, &pbegin, &outs, &parts, &total))
+      !upload_io_tables(specs, bg->ws.batch, cf, n, render_of, bg, &pjobs, &pbegin, &pblock_job, &outs, &parts,
+                        &total))
     return false;
   if (capture) {
     if (!bg->capture && !hip_ok(hipStreamCreateWithFlags(&bg->capture, hipStreamNonBlocking), "hipStreamCreate"))
@@ -627,8 +635,8 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
     const VisConfig* c0 = cf[0];
     const int nkeys = static_cast<int>(bg->ws.batch.keys.size());
     const bool fused = nkeys <= 256;  // the prep launch clears the work-queue counters
-    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, static_cast<int>(specs.size()), total, q,
-                                 fused ? bg->ws.batch.counters : nullptr, fused ? nkeys : 0),
+    ok = hip_ok(launch_prep_jobs(pjobs, pbegin, pblock_job, total, q, fused ? bg->ws.batch.counters : nullptr,
+                                 fused ? nkeys : 0),
                 "prep launch") &&
          mfma_launch_sums(specs, bg->ws.batch, q, nullptr, fused) &&
          hip_ok(launch_batch_finish(outs, parts, n, c0->_cutoff_frequency.x, c0->_cutoff_frequency.y, q),
