@@ -436,8 +436,14 @@ static bool bin_setting(const GroupInfo& info, int kind, const VisConfig* c, int
       return false;
     }
     // Defaults: 2 Chebyshev nodes per bin of ratio 1.01 (the two-node kernel; interpolation
-    // error < 1.2e-5 of Phi(0) pointwise), 3 nodes: 1.04 (< 3e-6), more: 1.1.
-    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? 1.01 : T == 3 ? 1.04 : 1.1);
+    // error < 1.2e-5 of Phi(0) pointwise), 3 nodes: 1.04 (< 3e-6), more: 1.1.  A group of fewer
+    // than kFewParticles particles takes two-node bins of ratio 1.0025 (16 times smaller error):
+    // with a few particles per bin the error does not average out (2 to 70 particles, every one
+    // at a bin edge: 0.9-1.5e-5 against FP64 at ratio 1.01, 0.7-1.1e-6 at 1.0025), and the
+    // narrower bins cost such a group nothing measurable.
+    constexpr int kFewParticles = 1024;
+    const double t2 = info.n < kFewParticles ? 1.0025 : 1.01;
+    const double ratio = c->h_bin_ratio > 1.0f ? c->h_bin_ratio : (T == 2 ? t2 : T == 3 ? 1.04 : 1.1);
     nb = c->number_of_h_bins > 0 ? c->number_of_h_bins : auto_bins(info.h_min, info.h_max, ratio);
   }
   // Radii of 0 take a bin of their own (bin 0, width 0) ahead of the log-spaced bins of the

@@ -24,10 +24,10 @@ def samples_for(cutoff):
     return (1 << (m - 1).bit_length(),) * 2
 
 
-def render(groups, cutoff, nodes, tile="default", form="mfma_split_f16"):
+def render(groups, cutoff, nodes, tile="default", form="mfma_split_f16", ratio=0.0):
     ds = host.Dataset(groups)
     with host.Config(cutoff=cutoff, samples=samples_for(cutoff), step=STEP, u=U, v=V, block_length=64,
-                     form=form, phi_mode="analytic", nodes=nodes, tile=tile) as cfg:
+                     form=form, phi_mode="analytic", nodes=nodes, tile=tile, h_bin_ratio=ratio) as cfg:
         assert cfg.check()
         cfg.register(ds)
         try:
@@ -130,3 +130,32 @@ def test_unknown_tile_is_refused():
     with host.Config(cutoff=(8, 8), samples=(64, 64), block_length=64, form="mfma_split_f16") as cfg:
         cfg.cs.tile = 4
         assert not cfg.check()
+
+
+@pytest.mark.parametrize("ratio", [0.0, 1.01])
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 33, 70])
+def test_few_particles_two_node(n, ratio):
+    """Two-node kernel items of 1-5 stages (a stage is one bin of at most 32 particles: n
+    particles with distinct radii fill n bins; 33 and 70 particles of one radius fill 2 and 3
+    stages of one bin), which end the producers' stage loop at each of its exits.  With so few
+    particles every one sits at an edge of its h-bin, where the two-node interpolation error
+    is largest, and nothing averages it out: bins of ratio 1.01 put these spectra 0.9-1.5e-5
+    from FP64, so groups under 1024 particles default to bins of ratio 1.0025 (0.7-1.1e-6);
+    at either the 1e-5 bar also pins the loop itself (a stage lost or repeated is an O(1)
+    error).  Ratio 1.01 is set explicitly here to check the larger groups' setting (bar 2e-5)."""
+    rng = np.random.default_rng(40 + n)
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    w = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    if n <= 5:
+        h = (0.02 * 1.5 ** np.arange(n)).astype(np.float32)
+    else:
+        h = np.full(n, 0.03, np.float32)
+        h[0] = 0.02  # two radii: binned, not the uniform-radius kernel
+    groups = [(pos, w, h, "WENDLAND_D3_C2")]
+    cutoff = (80, 70)
+    print(f"[case] few particles n={n}", flush=True)
+    F, img = render(groups, cutoff, 2, ratio=ratio)
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic")
+    e = oracle.rel_l2(F, F64)
+    print(f"ratio {ratio} rel-L2 {e:.2e}")
+    assert e <= (2e-5 if ratio else 1e-5)
