@@ -178,6 +178,12 @@ __device__ __forceinline__ float minus_half(float a, uint32_t hi) {
   return d;
 }
 
+// A wave-uniform value read through the scalar data cache (s_load: no vector registers, no
+// wait on the vector memory counter).  A read only; p must be the same in every lane.
+__device__ __forceinline__ float uniform_load(const float* p) {
+  return *(const __attribute__((address_space(4))) float*)p;
+}
+
 // The FP16 pair sum x + y of two packed words (v_pk_add_f16, round to nearest even).
 __device__ __forceinline__ uint32_t add_f16x2(uint32_t x, uint32_t y) {
   using h2 = _Float16 __attribute__((ext_vector_type(2)));
