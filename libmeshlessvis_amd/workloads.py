@@ -164,7 +164,7 @@ def lognormal_positions(n: int, grid: int, rng: np.random.Generator, slope: floa
     nz = kk > 0
     amp[nz] = kk[nz] ** (slope / 2.0)
     noise = rng.normal(size=kk.shape) + 1j * rng.normal(size=kk.shape)
-    delta = np.fft.irfftn(noise * amp, s=(grid, grid, grid))
+    delta = np.fft.irfftn(noise * amp, s=(grid, grid, grid), axes=(0, 1, 2))
     delta = (delta - delta.mean()) / delta.std()
     rho = np.exp(delta).ravel()
     p = rho / rho.sum()
