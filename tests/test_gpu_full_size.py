@@ -1,8 +1,8 @@
 """GPU parity at the full workload sizes, through the C ABI.
 
-Workloads: 262,144 clustered particles (Wendland, kNN(32) radii, 256^2), 1,048,576 uniform
-particles with signed weights (Gaussian h = 0.01, 512^2) and 4,194,304 particles of a log-normal
-field (Gaussian, kNN(48) radii, 1024^2).  The FP64 oracle over a whole half plane is out of reach
+Workloads: 262,144 clustered particles (Wendland, kNN(32) radii, 256^2; the same cloud also with
+the SPH basis), 1,048,576 uniform particles with signed weights (Gaussian h = 0.01, 512^2) and
+4,194,304 particles of a log-normal field (Gaussian, kNN(48) radii, 1024^2).  The FP64 oracle over a whole half plane is out of reach
 here (up to 2.2e12 terms per view), so every render is checked
 
   * on a stratified frequency sample against the FP64 oracle: every frequency with |x|, y <= 8
@@ -17,6 +17,7 @@ Tolerances (relative L2 against the FP64 oracle of the same Phi): plain FP32 and
 <= 1e-5 (the accuracy target), hi/lo split BF16 <= 3e-5, single FP16 <= 5e-3; group additivity
 <= 1e-5 for split FP16; C2R of the GPU's own spectrum <= 3e-6.
 """
+import dataclasses
 import functools
 
 import numpy as np
@@ -35,6 +36,8 @@ TOL = {"plain": 1e-5, "mfma_split_f16": 1e-5, "mfma_split_bf16": 3e-5, "mfma_f16
 def workload(name):
     if name == "clustered_wendland_262144":
         return W.clustered_wendland()
+    if name == "clustered_sph_262144":  # the same particles, weights and radii; M4 spline basis
+        return dataclasses.replace(workload("clustered_wendland_262144"), name=name, basis="SPH")
     if name == "uniform_signed_gaussian_1048576":
         return W.uniform_signed_gaussian()
     return W.lognormal_field()
@@ -84,6 +87,8 @@ CASES = [
     ("uniform_signed_gaussian_1048576", 0, "mfma_f16"),
     ("uniform_signed_gaussian_1048576", 0, "plain"),
     ("clustered_wendland_262144", 0, "mfma_split_bf16"),
+    ("clustered_sph_262144", 0, "mfma_split_f16"),
+    ("clustered_sph_262144", 0, "plain"),
     ("lognormal_field_4194304", 0, "mfma_split_f16"),
     ("lognormal_field_4194304", 3, "mfma_split_f16"),
     ("lognormal_field_4194304", 0, "mfma_split_bf16"),
@@ -129,13 +134,14 @@ def test_full_size_group_additivity(name):
 # the oracle's port: same loop, FP32 phase 2*pi*dot(k, p), serial FP32 sums) on the same inputs, at
 # a stratified frequency sample.  e_ref = port vs FP64, e_build = GPU vs FP64, e_par = GPU vs port;
 # parity holds when e_par <= e_ref + e_build + 1e-6.  Reference Phi (the reference's formulas,
-# exp(+pi r^2) for the Gaussian): the clustered Wendland set and the uniform Gaussian set (h = 0.01,
+# exp(+pi r^2) for the Gaussian): the clustered set (Wendland and SPH) and the uniform Gaussian set (h = 0.01,
 # r h <= 1.81, exp(+pi r^2 h^2) <= 3e4).  The log-normal field's Gaussian radii reach r h ~ 36, where
 # exp(+pi r^2 h^2) overflows FP32: it is compared in analytic mode, with the port's Phi swapped for
 # the exact transform (the reference's loop, phase and FP32 sums kept).
 REF_CASES = [
     ("clustered_wendland_262144", "reference", "plain"),
     ("clustered_wendland_262144", "reference", "mfma_split_f16"),
+    ("clustered_sph_262144", "reference", "mfma_split_f16"),
     ("uniform_signed_gaussian_1048576", "reference", "plain"),
     ("uniform_signed_gaussian_1048576", "reference", "mfma_split_f16"),
     ("clustered_wendland_262144", "analytic", "mfma_split_f16"),
