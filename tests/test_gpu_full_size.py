@@ -66,11 +66,11 @@ def oracle_sample(name, view):
     return idx, weight, F64
 
 
-def render(wl, groups, view, form, phi_mode=None):
+def render(wl, groups, view, form, phi_mode=None, **settings):
     vw = wl.views[view]
     ds = host.Dataset(groups)
     with host.Config(cutoff=wl.cutoff, samples=wl.samples, step=wl.step, u=vw.u, v=vw.v, form=form,
-                     phi_mode=phi_mode or wl.phi_mode) as cfg:
+                     phi_mode=phi_mode or wl.phi_mode, **settings) as cfg:
         cfg.register(ds)
         try:
             cfg.render(ds)
@@ -113,6 +113,21 @@ def test_full_size_sampled_parity(name, view, form):
     e_img = oracle.rel_l2(img, ref)
     print(f"{name} view {view} {form}: image vs irfft2 of its spectrum {e_img:.3e}")
     assert e_img <= 3e-6
+
+
+@pytest.mark.parametrize("name", ["clustered_wendland_262144", "clustered_sph_262144"])
+@pytest.mark.parametrize("nodes", [3, 4])
+def test_full_size_more_nodes(name, nodes):
+    """The binned kernel with 3 or 4 Chebyshev nodes per h-bin (default bin ratios 1.04 and 1.1),
+    the setting vis_autotune may choose, against the FP64 oracle on the stratified sample."""
+    print(f"[case] {name} {nodes} nodes per bin", flush=True)
+    wl = workload(name)
+    F, _ = render(wl, [wl.group()], 0, "mfma_split_f16", nodes=nodes)
+    assert np.isfinite(F).all()
+    idx, weight, F64 = oracle_sample(name, 0)
+    e = oracle.sampled_rel_l2(F, F64, idx, weight)
+    print(f"{name} {nodes} nodes: sampled rel-L2 vs FP64 {e:.3e} ({idx.size} frequencies)")
+    assert e <= TOL["mfma_split_f16"]
 
 
 @pytest.mark.parametrize("name", ["clustered_wendland_262144", "lognormal_field_4194304"])
