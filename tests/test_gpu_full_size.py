@@ -130,6 +130,21 @@ def test_full_size_more_nodes(name, nodes):
     assert e <= TOL["mfma_split_f16"]
 
 
+@pytest.mark.parametrize("name", ["clustered_wendland_262144", "uniform_signed_gaussian_1048576"])
+@pytest.mark.parametrize("tile", ["half_x", "half_y", "quarter"])
+def test_full_size_tile_variants(name, tile):
+    """The output-tile variants vis_autotune may choose (two-node kernel on the clustered cloud,
+    sum kernel on the uniform one) against the FP64 oracle on the stratified sample."""
+    print(f"[case] {name} tile {tile}", flush=True)
+    wl = workload(name)
+    F, _ = render(wl, [wl.group()], 0, "mfma_split_f16", tile=tile)
+    assert np.isfinite(F).all()
+    idx, weight, F64 = oracle_sample(name, 0)
+    e = oracle.sampled_rel_l2(F, F64, idx, weight)
+    print(f"{name} tile {tile}: sampled rel-L2 vs FP64 {e:.3e} ({idx.size} frequencies)")
+    assert e <= TOL["mfma_split_f16"]
+
+
 @pytest.mark.parametrize("name", ["clustered_wendland_262144", "lognormal_field_4194304"])
 def test_full_size_group_additivity(name):
     """Every frequency: the dataset as two groups (a random half each) against it as one group."""
