@@ -159,3 +159,68 @@ def test_few_particles_two_node(n, ratio):
     e = oracle.rel_l2(F, F64)
     print(f"ratio {ratio} rel-L2 {e:.2e}")
     assert e <= (2e-5 if ratio else 1e-5)
+
+
+
+def large_case(kind, cutoff):
+    basis, nodes = KINDS.get(kind, ("GAUSSIAN", 0))
+    rng = np.random.default_rng(cutoff[0] + len(kind))
+    n = 8192
+    pos = rng.uniform(-0.5, 0.5, (n, 3)).astype(np.float32)
+    w = rng.uniform(-1.0, 1.0, n).astype(np.float32)
+    h = np.exp(rng.uniform(np.log(0.002), np.log(0.01), n)).astype(np.float32)
+    idx, weight = oracle.stratified_sample(cutoff, np.random.default_rng(5), n_random=256)
+    return [(pos, w, h, basis)], nodes, idx, weight
+
+
+@pytest.mark.parametrize("kind", ["gauss_var", "plain"])
+@pytest.mark.parametrize("cutoff", [(1024, 1024), (2048, 2048)])
+def test_large_images(kind, cutoff):
+    """2048^2 and 4096^2 images (2.1M and 8.4M half-plane frequencies) of 8192 uniformly placed
+    particles: the spectrum on a stratified sample against the FP64 oracle (<= 1e-5), the image
+    against numpy's irfft2 of the arranged spectrum on every pixel (<= 3e-6)."""
+    form = "plain" if kind == "plain" else "mfma_split_f16"
+    groups, nodes, idx, weight = large_case(kind, cutoff)
+    print(f"[case] {kind} cutoff={cutoff} image {samples_for(cutoff)}", flush=True)
+    F, img = render(groups, cutoff, nodes, form=form)
+    assert np.isfinite(F).all() and np.isfinite(img).all()
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic", idx=idx)
+    e = oracle.sampled_rel_l2(F, F64, idx, weight)
+    ref = np.fft.irfft2(oracle.arrange64(F.astype(np.complex128), *cutoff, *samples_for(cutoff)),
+                        s=samples_for(cutoff), norm="forward")
+    e_img = oracle.rel_l2(img, ref)
+    print(f"{kind} cutoff={cutoff}: sampled rel-L2 vs FP64 {e:.3e}, image vs irfft2 {e_img:.3e}")
+    assert e <= 1e-5
+    assert e_img <= 3e-6
+
+
+@pytest.mark.parametrize("cutoff", [(1024, 1024), (2048, 2048)])
+def test_large_images_wendland_autotuned(cutoff):
+    """The same uniformly placed particles with the Wendland basis: their spectrum keeps its energy
+    up to the highest frequencies, where r h reaches 7-14 and the two-node interpolation error,
+    which grows with (r h (ratio - 1))^2, is largest.  The default bins (ratio 1.01) are held to
+    3e-5 here; vis_autotune at tolerance 1e-5, which measures every candidate against the plain
+    kernel over the whole plane, must find a setting within 1e-5 of the FP64 oracle."""
+    groups, _, idx, weight = large_case("wendland_t2", cutoff)
+    F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic", idx=idx)
+    ds = host.Dataset(groups)
+    with host.Config(cutoff=cutoff, samples=samples_for(cutoff), step=STEP, u=U, v=V, block_length=64,
+                     form="mfma_split_f16", phi_mode="analytic") as cfg:
+        cfg.register(ds)
+        try:
+            print(f"[case] wendland default bins cutoff={cutoff}", flush=True)
+            cfg.render(ds)
+            e_default = oracle.sampled_rel_l2(cfg.spectrum(), F64, idx, weight)
+            print(f"[case] wendland autotune cutoff={cutoff}", flush=True)
+            cfg.autotune(ds, 1e-5)
+            cfg.render(ds)
+            F = cfg.spectrum()
+            e_tuned = oracle.sampled_rel_l2(F, F64, idx, weight)
+            chosen = (cfg.cs.number_of_interpolation_nodes, cfg.cs.h_bin_ratio, cfg.cs.tile)
+        finally:
+            cfg.unregister(ds)
+    print(f"wendland cutoff={cutoff}: default bins {e_default:.3e}; autotuned (nodes, ratio, tile) "
+          f"{chosen}: {e_tuned:.3e}")
+    assert np.isfinite(F).all()
+    assert e_default <= 3e-5
+    assert e_tuned <= 1e-5
