@@ -122,7 +122,11 @@ typedef struct
 	int number_of_h_bins;               /* MFMA form, SPH/Wendland radii: 0 = auto     */
 	int number_of_interpolation_nodes;  /* MFMA form, SPH/Wendland radii: nodes per bin, 0 = auto */
 	float h_bin_ratio;                  /* MFMA form: ratio h_hi/h_lo of the radius bins, 0 = auto
-	                                       (used when number_of_h_bins is 0)                         */
+	                                       (used when number_of_h_bins is 0).  Auto for SPH/Wendland:
+	                                       2 nodes, ratio 1.01 (1.0025 below 1024 particles); the
+	                                       interpolation error grows with (|f| h (ratio - 1))^2, so
+	                                       spectra with much of their energy at |f| h above ~5 want
+	                                       narrower bins: vis_autotune measures and picks them.     */
 	int tile;                           /* MFMA form: enum VisTile, default VIS_TILE_DEFAULT      */
 } VisConfig;
 
