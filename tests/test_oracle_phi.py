@@ -10,6 +10,7 @@ Each check is independent of the closed forms being tested:
     deviation from the exact transform below the switch.
 """
 import math
+import warnings
 
 import numpy as np
 import pytest
@@ -44,7 +45,11 @@ def quad_phi(basis, k):
                                 points=[1.0] if basis == "SPH" else None)
         return val
     f = lambda r: r * phi(r)
-    val, _ = integrate.quad(f, 0, R, weight="sin", wvar=2 * math.pi * k, limit=800, epsabs=1e-15, epsrel=1e-13)
+    # Where the transform is far below epsabs (the Gaussian at large k) QUADPACK reports roundoff
+    # before reaching it; the comparisons below bound the error absolutely, so that is harmless.
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore", integrate.IntegrationWarning)
+        val, _ = integrate.quad(f, 0, R, weight="sin", wvar=2 * math.pi * k, limit=800, epsabs=1e-15, epsrel=1e-13)
     return 2.0 / k * val
 
 
