@@ -199,8 +199,9 @@ def test_large_images_wendland_autotuned(cutoff):
     """The same uniformly placed particles with the Wendland basis: their spectrum keeps its energy
     up to the highest frequencies, where r h reaches 7-14 and the two-node interpolation error,
     which grows with (r h (ratio - 1))^2, is largest.  The default bins (ratio 1.01) are held to
-    3e-5 here; vis_autotune at tolerance 1e-5, which measures every candidate against the plain
-    kernel over the whole plane, must find a setting within 1e-5 of the FP64 oracle."""
+    3e-5 here; vis_autotune, which measures every candidate against the plain kernel over the
+    whole plane, at tolerance 5e-6 (half the target: margin for the sampled check and for which
+    candidate is fastest on the box) must find a setting within 1e-5 of the FP64 oracle."""
     groups, _, idx, weight = large_case("wendland_t2", cutoff)
     F64 = oracle.spectrum64(groups, cutoff, STEP, U, V, "analytic", idx=idx)
     ds = host.Dataset(groups)
@@ -212,7 +213,7 @@ def test_large_images_wendland_autotuned(cutoff):
             cfg.render(ds)
             e_default = oracle.sampled_rel_l2(cfg.spectrum(), F64, idx, weight)
             print(f"[case] wendland autotune cutoff={cutoff}", flush=True)
-            cfg.autotune(ds, 1e-5)
+            cfg.autotune(ds, 5e-6)
             cfg.render(ds)
             F = cfg.spectrum()
             e_tuned = oracle.sampled_rel_l2(F, F64, idx, weight)
