@@ -190,6 +190,14 @@ void vis_register_meshless_dataset_async(VisConfig* vis_config, MeshlessDataset*
    unchanged; otherwise the renders run one after the other. */
 void vis_fourier_volume_rendering_batched(MeshlessDataset** meshless_datasets, VisConfig** vis_configs, int number_of_renders);
 
+/* Device memory the MFMA forms may hold at once in FP32 partial spectra (one per chunk of
+   up to 16,384 particles: 8 B per half-plane frequency each), process-wide; <= 0 restores
+   the default, 8 GiB.  A render or batch needing more runs in passes over its chunks, each
+   reduced into the FP64 accumulator before the next reuses the memory, so the particle
+   count is bounded by the particles' own storage rather than by the partials.  Takes effect
+   at the next plan (a changed dataset, grid or setting, or this call). */
+void vis_set_partial_budget(long long bytes);
+
 /* No counterpart in the reference.  For callers that cannot allocate device memory
    (code under HIP graph capture, tight serving loops, harnesses that forbid it): the
    render of vis_fourier_volume_rendering for a config with form VIS_FORM_PLAIN,

@@ -131,6 +131,7 @@ PROTOTYPES = {
     "vis_copy_spectrum_to_host": (None, [VisConfigP, C.POINTER(float2)]),
     "vis_register_meshless_dataset_async": (None, [VisConfigP, DatasetP]),
     "vis_debug_fail_device_allocations": (None, [C.c_int]),
+    "vis_set_partial_budget": (None, [C.c_longlong]),
     "vis_debug_batch_graph_counts": (None, [VisConfigP, C.POINTER(C.c_int), C.POINTER(C.c_longlong)]),
     "vis_fourier_volume_rendering_batched": (None, [C.POINTER(DatasetP), C.POINTER(VisConfigP), C.c_int]),
     "vis_fourier_volume_rendering_noalloc": (None, [DatasetP, VisConfigP]),

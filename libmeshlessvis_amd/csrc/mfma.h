@@ -29,14 +29,19 @@ struct MfmaJobSpec {
 // Host plan of one job.
 struct MfmaPlanned;
 
-// Work tables of a prepared batch (device pointers into the workspace).
+// Work tables of a prepared batch (device pointers into the workspace).  A batch whose chunk
+// slots would exceed the partial budget runs in passes: pass p launches every kernel variant over
+// the items of its share of each job's chunks (slots rebased to 0) and reduces them into the FP64
+// accumulators before the next pass reuses the slots.
 struct MfmaBatch {
   std::vector<MfmaPlanned> plans;
   std::vector<std::tuple<int, int, int, int>> keys;  // (kind, basis, nodes, tile) of every launch
-  std::vector<int> job_begin, item_begin;
+  std::vector<int> job_begin;   // jobs of key k: [job_begin[k], job_begin[k + 1])
+  std::vector<int> item_begin;  // items of pass p, key k: [item_begin[p K + k], item_begin[p K + k + 1])
+  int passes = 1;
   const MfmaJob* jobs = nullptr;
   const MfmaItem* items = nullptr;
-  int* counters = nullptr;
+  int* counters = nullptr;  // one per (pass, key), p K + k
 };
 
 // Device buffers the MFMA forms keep between calls.
@@ -66,9 +71,13 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
 // Only the work-queue reset (unless a prep launch cleared the counters) and the persistent MFMA
 // kernels (packets prepared, partials reduced by the caller).
 bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev,
-                      bool counters_cleared = false);
+                      bool counters_cleared = false, int pass = 0);
 // prepare + enqueue.
 bool mfma_execute(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWorkspace* ws, EventPair* ev);
+// Bytes of FP32 partial spectra a batch may hold at once before it runs in passes (<= 0: the
+// default, kDefaultPartialBudget).  Takes effect at the next plan.
+constexpr long long kDefaultPartialBudget = 8LL << 30;
+void mfma_set_partial_budget(long long bytes);
 
 // A planned job's particles and radii in slot order (nullptr: slot i holds particle i of the
 // group, zeros past n).

@@ -38,6 +38,7 @@
 //    tile (FP32) is reduced with the job's other items in FP64, in a fixed order.
 #include "mfma_common.h"
 
+#include <atomic>
 #include <cstring>
 #include <type_traits>
 #include <tuple>
@@ -643,14 +644,21 @@ bool mfma_plan(const MfmaJobSpec& spec, hipStream_t s, long long target_steps, M
                   std::min(bin_step[b + 1], s0 + chunk_steps) - s0);
   }
   if (chunk == 0) add_range(0, 0, 0, 0);
-  out->nchunks = paired ? 2 * chunk : chunk;  // the two-node kernel writes one slot per node
+  out->item_chunks = chunk;
+  out->slots_per_chunk = paired ? 2 : 1;  // the two-node kernel writes one slot per node
+  out->nchunks = out->slots_per_chunk * chunk;
+  out->pass_chunks = chunk;
   return true;
 }
+
+static std::atomic<long long> g_partial_budget{kDefaultPartialBudget};
+void mfma_set_partial_budget(long long bytes) { g_partial_budget = bytes > 0 ? bytes : kDefaultPartialBudget; }
 
 // Host half of a batch: plans, buffer sizes, work tables uploaded (synchronous).
 static std::vector<unsigned long long> plan_key(const std::vector<MfmaJobSpec>& specs) {
   std::vector<unsigned long long> k;
   k.push_back(specs.size());
+  k.push_back(static_cast<unsigned long long>(g_partial_budget.load()));
   for (const MfmaJobSpec& sp : specs) {
     uint32_t ratio;
     std::memcpy(&ratio, &sp.c->h_bin_ratio, sizeof(ratio));
@@ -704,7 +712,7 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
     work += bt->plans[k].tile_steps;
   }
   const long long target = std::max<long long>(1, work / (16LL * device_cus()));
-  size_t slots = 0, parts = 0;
+  size_t slots = 0, full = 0;
   for (size_t k = 0; k < specs.size(); ++k) {
     if (specs.size() > 1) {
       if (first[k] != k)
@@ -713,10 +721,30 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
         return false;
     }
     bt->plans[k].pk_off = slots;
-    bt->plans[k].part_off = parts;
     slots += bt->plans[k].n_slots;
-    parts += static_cast<size_t>(bt->plans[k].nchunks) * 2 * specs[k].vp.cx * specs[k].vp.cy;
+    full += static_cast<size_t>(bt->plans[k].nchunks) * 2 * specs[k].vp.cx * specs[k].vp.cy;
   }
+  // The partial slots grow with the particle count (one spectrum per 16,384 particles of the sum
+  // kernel): past the budget the batch runs in passes, each over a 1/passes share of every job's
+  // chunks (at least one), its slots reduced into the FP64 accumulators before the next pass.
+  const size_t budget = static_cast<size_t>(g_partial_budget.load()) / sizeof(float2);
+  int passes = 1;
+  if (full > budget) {
+    const size_t want = (full + budget - 1) / budget;
+    int most = 1;
+    for (const MfmaPlanned& pl : bt->plans) most = std::max(most, pl.item_chunks);
+    passes = static_cast<int>(std::min<size_t>(want, static_cast<size_t>(most)));
+  }
+  size_t parts = 0;
+  int used = 1;
+  for (size_t k = 0; k < specs.size(); ++k) {
+    MfmaPlanned& pl = bt->plans[k];
+    pl.pass_chunks = (pl.item_chunks + passes - 1) / passes;
+    used = std::max(used, (pl.item_chunks + pl.pass_chunks - 1) / pl.pass_chunks);
+    pl.part_off = parts;
+    parts += static_cast<size_t>(pl.pass_chunks) * pl.slots_per_chunk * 2 * specs[k].vp.cx * specs[k].vp.cy;
+  }
+  bt->passes = used;
   if (!ws->packets.reserve(sizeof(Packet) * slots) || !ws->partial.reserve(sizeof(float2) * std::max<size_t>(parts, 1)))
     return false;
   for (auto& pl : bt->plans) {
@@ -730,31 +758,44 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   for (auto& pl : bt->plans)
     if (std::find(bt->keys.begin(), bt->keys.end(), key_of(pl)) == bt->keys.end()) bt->keys.push_back(key_of(pl));
   std::vector<MfmaJob> jobs;
-  std::vector<MfmaItem> items;
+  std::vector<int> jid(bt->plans.size(), 0);  // a plan's job index within its key's jobs
   for (auto& key : bt->keys) {
     bt->job_begin.push_back(static_cast<int>(jobs.size()));
-    bt->item_begin.push_back(static_cast<int>(items.size()));
-    std::vector<MfmaItem> its;
-    for (auto& pl : bt->plans) {
-      if (key_of(pl) != key) continue;
-      const int jid = static_cast<int>(jobs.size()) - bt->job_begin.back();
-      jobs.push_back(pl.job);
-      for (MfmaItem it : pl.items) {
-        it.job = jid;
-        its.push_back(it);
-      }
+    for (size_t j = 0; j < bt->plans.size(); ++j) {
+      if (key_of(bt->plans[j]) != key) continue;
+      jid[j] = static_cast<int>(jobs.size()) - bt->job_begin.back();
+      jobs.push_back(bt->plans[j].job);
     }
-    // Costliest items first: the atomic queue then balances the tail.
-    std::stable_sort(its.begin(), its.end(), [](const MfmaItem& a, const MfmaItem& b) { return a.cost > b.cost; });
-    items.insert(items.end(), its.begin(), its.end());
   }
   bt->job_begin.push_back(static_cast<int>(jobs.size()));
+  std::vector<MfmaItem> items;
+  for (int p = 0; p < bt->passes; ++p) {
+    for (auto& key : bt->keys) {
+      bt->item_begin.push_back(static_cast<int>(items.size()));
+      std::vector<MfmaItem> its;
+      for (size_t j = 0; j < bt->plans.size(); ++j) {
+        const MfmaPlanned& pl = bt->plans[j];
+        if (key_of(pl) != key) continue;
+        const int c0 = p * pl.pass_chunks;
+        for (MfmaItem it : pl.items) {
+          if (it.chunk < c0 || it.chunk >= c0 + pl.pass_chunks) continue;
+          it.job = jid[j];
+          it.chunk -= c0;  // the pass's own slots
+          its.push_back(it);
+        }
+      }
+      // Costliest items first: the atomic queue then balances the tail.
+      std::stable_sort(its.begin(), its.end(), [](const MfmaItem& a, const MfmaItem& b) { return a.cost > b.cost; });
+      items.insert(items.end(), its.begin(), its.end());
+    }
+  }
   bt->item_begin.push_back(static_cast<int>(items.size()));
   const size_t jbytes = sizeof(MfmaJob) * jobs.size(), ibytes = sizeof(MfmaItem) * items.size();
   // The previous upload (and the launches reading the previous tables) must be done before
   // the host copy and the device tables are reused.
   if (!hip_ok(hipStreamSynchronize(s), "synchronize")) return false;
-  if (!ws->tables.reserve(jbytes + ibytes) || !ws->counters.reserve(sizeof(int) * bt->keys.size())) return false;
+  if (!ws->tables.reserve(jbytes + ibytes) || !ws->counters.reserve(sizeof(int) * bt->passes * bt->keys.size()))
+    return false;
   ws->tables_host.resize(jbytes + ibytes);
   std::memcpy(ws->tables_host.data(), jobs.data(), jbytes);
   std::memcpy(ws->tables_host.data() + jbytes, items.data(), ibytes);
@@ -778,13 +819,13 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
   if (specs.empty()) return true;
   // The first packet prep also clears the work-queue counters (a prep launch that clears
   // counters runs at least one block), one launch fewer per render.
-  const int nkeys = static_cast<int>(bt.keys.size());
-  const bool fused = nkeys <= 256;  // one block clears them
+  const int ncounters = bt.passes * static_cast<int>(bt.keys.size());
+  const bool fused = ncounters <= 256;  // one block clears them
   for (size_t k = 0; k < specs.size(); ++k) {
     const MfmaJobSpec& sp = specs[k];
     const MfmaPlanned& pl = bt.plans[k];
     int* zero = k == 0 && fused ? bt.counters : nullptr;
-    const int nzero = k == 0 && fused ? nkeys : 0;
+    const int nzero = k == 0 && fused ? ncounters : 0;
     hipError_t e;
     if (pl.bc)  // the binned particles in slot order: every slot read in place
       e = launch_prep_packets(pl.bc->ordered.as<Constraint>(), pl.bc->ordered_r.as<float>(), pl.n_slots, pl.n_slots,
@@ -797,21 +838,35 @@ bool mfma_enqueue(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hi
                               const_cast<Packet*>(pl.job.packets), s, zero, nzero);
     if (!hip_ok(e, "prep_packets launch")) return false;
   }
-  if (!mfma_launch_sums(specs, bt, s, ev, fused)) return false;
-  for (size_t k = 0; k < specs.size(); ++k) {
-    const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
-    hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(256), 0, s, bt.plans[k].job.partial,
-                       bt.plans[k].nchunks, specs[k].vp.cx, specs[k].vp.cy, specs[k].first ? 1 : 0, specs[k].acc);
-    if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
+  // In passes, the timed span (ev) runs from the first pass's kernels to the last pass's and
+  // includes the reductions between them.
+  const bool one = bt.passes == 1;
+  if (ev && !one && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
+  for (int p = 0; p < bt.passes; ++p) {
+    if (!mfma_launch_sums(specs, bt, s, one ? ev : nullptr, fused, p)) return false;
+    if (ev && !one && p + 1 == bt.passes && !hip_ok(hipEventRecord(ev->second, s), "hipEventRecord")) return false;
+    for (size_t k = 0; k < specs.size(); ++k) {
+      const MfmaPlanned& pl = bt.plans[k];
+      const int chunks = std::min(pl.pass_chunks, pl.item_chunks - p * pl.pass_chunks);
+      if (chunks <= 0) continue;  // this job's chunks all ran in earlier passes
+      const int nfreq = 2 * specs[k].vp.cx * specs[k].vp.cy;
+      const bool assign = p == 0 && specs[k].first;
+      hipLaunchKernelGGL(mfma_reduce_kernel, dim3((nfreq + 63) / 64), dim3(256), 0, s, pl.job.partial,
+                         chunks * pl.slots_per_chunk, specs[k].vp.cx, specs[k].vp.cy, assign ? 1 : 0, specs[k].acc);
+      if (!hip_ok(hipGetLastError(), "mfma_reduce launch")) return false;
+    }
   }
   return true;
 }
 
 bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt, hipStream_t s, EventPair* ev,
-                      bool counters_cleared) {
+                      bool counters_cleared, int pass) {
   if (specs.empty()) return true;
+  if (pass < 0 || pass >= bt.passes) return hip_ok(hipErrorInvalidValue, "mfma launch: pass out of range");
   const VisConfig* c0 = specs[0].c;
-  if (!counters_cleared && !hip_ok(launch_zero_words(bt.counters, bt.keys.size(), s), "zero counters")) return false;
+  const size_t nkeys = bt.keys.size();
+  int* counters = bt.counters + static_cast<size_t>(pass) * nkeys;
+  if (!counters_cleared && !hip_ok(launch_zero_words(counters, nkeys, s), "zero counters")) return false;
 
   MfmaShared sh{};
   sh.cx = specs[0].vp.cx;
@@ -822,15 +877,17 @@ bool mfma_launch_sums(const std::vector<MfmaJobSpec>& specs, const MfmaBatch& bt
   const float sigma = (mode == VIS_PHI_REFERENCE) ? 1.0f : -1.0f;
   sh.ref_hi = sigma > 0.0f ? 1 : 0;
   if (ev && !hip_ok(hipEventRecord(ev->first, s), "hipEventRecord")) return false;
-  for (size_t k = 0; k < bt.keys.size(); ++k) {
+  for (size_t k = 0; k < nkeys; ++k) {
     const int kind = std::get<0>(bt.keys[k]), basis = std::get<1>(bt.keys[k]), T = std::get<2>(bt.keys[k]);
+    const size_t ib = static_cast<size_t>(pass) * nkeys + k;
     sh.tile = std::get<3>(bt.keys[k]);
-    sh.n_items = bt.item_begin[k + 1] - bt.item_begin[k];
+    sh.n_items = bt.item_begin[ib + 1] - bt.item_begin[ib];
+    if (sh.n_items == 0) continue;  // (a later pass, every job of this variant done)
     sh.sigma_pi_log2e = kind == KIND_GAUSS ? sigma * kPi * kLog2e : 0.0f;
     const int grid = std::max(1, std::min(sh.n_items, device_cus()));
     const MfmaJob* kj = bt.jobs + bt.job_begin[k];
-    const MfmaItem* ki = bt.items + bt.item_begin[k];
-    int* ctr = bt.counters + k;
+    const MfmaItem* ki = bt.items + bt.item_begin[ib];
+    int* ctr = counters + k;
     if (kind == KIND_BINNED) {
       if (!launch_binned(grid, s, kj, ki, sh, ctr, c0->form, T, basis, mode)) return false;
     } else if (c0->form == VIS_FORM_MFMA_F16) {

@@ -29,7 +29,10 @@ struct MfmaPlanned {
   int kind = 0, basis = 0, T = 1;
   int tile = 0;  // enum VisTile of the kernel that runs the job
   const BinCache* bc = nullptr;
-  int n_slots = 0, nchunks = 0;
+  int n_slots = 0, nchunks = 0;  // nchunks: partial slots over all passes (two per chunk, two nodes)
+  int item_chunks = 0;           // distinct MfmaItem::chunk values
+  int slots_per_chunk = 1;       // 2 for the two-node kernel (one slot per node)
+  int pass_chunks = 0;           // chunks per pass (item_chunks when the batch runs in one pass)
   long long tile_steps = 0;  // steps x tiles: the job's work
   size_t pk_off = 0, part_off = 0;
   MfmaJob job{};

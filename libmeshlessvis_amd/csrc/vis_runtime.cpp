@@ -611,6 +611,10 @@ static bool enqueue_batch(MeshlessDataset** ds, VisConfig** cf, int n, BatchGrap
   }
   same_grid = same_grid && n <= 65535;  // render = grid row of the batched launches
   if (!mfma_prepare(specs, s, &bg->ws, &bg->ws.batch)) return false;
+  // A batch too large for its partial slots at once runs in passes, each reduced by
+  // mfma_enqueue; the batched finish reduces a single pass, so such a batch takes the
+  // per-render finish.
+  if (bg->ws.batch.passes > 1) same_grid = false;
   const PrepJob* pjobs = nullptr;
   const long long* pbegin = nullptr;
   const int* pblock_job = nullptr;
@@ -1367,6 +1371,12 @@ double vis_autotune(MeshlessDataset* ds, VisConfig* c, double tolerance) {
 }
 
 void vis_debug_fail_device_allocations(int after) { g_fail_after.store(after < 0 ? -1 : after); }
+
+void vis_set_partial_budget(long long bytes) {
+  const Range range("vis_set_partial_budget");
+  clear_error();
+  mfma_set_partial_budget(bytes);
+}
 
 void vis_debug_batch_graph_counts(const VisConfig* c, int* captures, long long* launches) {
   clear_error();
