@@ -682,6 +682,7 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   bt->keys.clear();
   bt->job_begin.clear();
   bt->item_begin.clear();
+  bt->passes = 1;
   if (specs.empty()) return true;
   // Items per batch: about 16 per CU over the whole batch, so a batch of many small
   // renders is not split more finely than it needs to be.
@@ -727,7 +728,7 @@ bool mfma_prepare(const std::vector<MfmaJobSpec>& specs, hipStream_t s, MfmaWork
   // The partial slots grow with the particle count (one spectrum per 16,384 particles of the sum
   // kernel): past the budget the batch runs in passes, each over a 1/passes share of every job's
   // chunks (at least one), its slots reduced into the FP64 accumulators before the next pass.
-  const size_t budget = static_cast<size_t>(g_partial_budget.load()) / sizeof(float2);
+  const size_t budget = std::max<size_t>(1, static_cast<size_t>(g_partial_budget.load()) / sizeof(float2));
   int passes = 1;
   if (full > budget) {
     const size_t want = (full + budget - 1) / budget;

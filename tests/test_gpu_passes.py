@@ -48,7 +48,7 @@ def render_spectrum(groups, nodes, cutoff, samples, vw, budget):
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("budget", [256 << 10, 1 << 20])  # partials of 0.7-24 MB in one pass
+@pytest.mark.parametrize("budget", [1, 256 << 10, 1 << 20])  # partials of 0.7-24 MB in one pass; 1: a pass per chunk
 def test_render_in_passes(name, budget):
     groups, nodes, cutoff, samples = cloud(name)
     vw = W.random_view(np.random.default_rng(5))
