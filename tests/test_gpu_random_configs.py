@@ -106,3 +106,54 @@ def test_random_config(seed):
     print(f"{desc}: rel-L2 vs FP64 {e:.2e}, image vs irfft2 {e_img:.2e}", flush=True)
     assert e <= BAR[form]
     assert e_img <= 3e-6
+
+
+@pytest.mark.parametrize("seed", range(100, 112))
+def test_random_batch(seed):
+    """A batch of 2-5 renders (random datasets and views) sharing one random grid and MFMA setting:
+    the first call (no graph), the capture and a replay give the same images bit for bit, each
+    within 1e-6 of its one-at-a-time render and its spectrum within the form's bar of FP64."""
+    rng = np.random.default_rng(seed)
+    _, cutoff, samples, step, form, nodes, tile, budget, _, _ = draw(seed)
+    if form == "plain":
+        form = "mfma_split_f16"
+    k = int(rng.integers(2, 6))
+    cases = [draw(seed * 1000 + i) for i in range(k)]
+    sets = [host.Dataset(c[0]) for c in cases]
+    vws = [c[9] for c in cases]
+    desc = f"seed {seed}: {k} renders, cutoff {cutoff} samples {samples} {form} nodes {nodes} tile {tile} budget {budget}"
+    print(f"[case] {desc}", flush=True)
+    cfgs = [host.Config(cutoff=cutoff, samples=samples, step=step, u=vw.u, v=vw.v, form=form, phi_mode="analytic",
+                        nodes=nodes, tile=tile, block_length=64) for vw in vws]
+    try:
+        abi.lib().vis_set_partial_budget(budget)
+        for ds in sets:
+            cfgs[0].register(ds)
+        single = []
+        for ds, cfg in zip(sets, cfgs):
+            cfg.render(ds)
+            single.append((cfg.image(), cfg.spectrum()))
+        first = None
+        for rep in range(3):
+            host.render_batched(sets, cfgs)
+            imgs = [cfg.image() for cfg in cfgs]
+            if first is None:
+                first = imgs
+                spectra = [cfg.spectrum() for cfg in cfgs]
+            else:
+                for a, b in zip(imgs, first):
+                    assert np.array_equal(a, b), (desc, rep)
+    finally:
+        abi.lib().vis_set_partial_budget(0)
+        for ds in sets:
+            cfgs[0].unregister(ds)
+        for cfg in cfgs:
+            cfg.close()
+    worst = 0.0
+    for (img1, _), img, F, c in zip(single, first, spectra, cases):
+        assert oracle.rel_l2(img, img1) <= 1e-6, desc
+        F64 = oracle.spectrum64(c[0], cutoff, step, c[9].u, c[9].v, "analytic")
+        e = oracle.rel_l2(F, F64)
+        worst = max(worst, e)
+        assert e <= BAR[form], desc
+    print(f"{desc}: batched vs single <= 1e-6, worst rel-L2 vs FP64 {worst:.2e}", flush=True)
