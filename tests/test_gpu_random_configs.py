@@ -26,12 +26,12 @@ BAR = {"plain": 1e-5, "mfma_split_f16": 1e-5, "mfma_split_bf16": 5e-5, "mfma_f16
 CULL_K = {"SPH": 2.0, "WENDLAND_D3_C2": 1.0, "GAUSSIAN": 3.0}
 
 
-def draw(seed):
+def draw(seed, bases=BASES):
     rng = np.random.default_rng(seed)
     groups = []
     for _ in range(int(rng.integers(1, 4))):
         n = int(rng.integers(1, 3001))
-        basis = BASES[int(rng.integers(0, 3))]
+        basis = bases[int(rng.integers(0, 3)) % len(bases)]
         pos = rng.uniform(-0.6, 0.6, (n, 3)).astype(np.float32)
         signed = rng.random() < 0.5
         w = rng.uniform(-1 if signed else 0.1, 1, n).astype(np.float32)
@@ -79,17 +79,20 @@ def culled(groups, step, vw):
     return out
 
 
-@pytest.mark.parametrize("seed", range(48))
-def test_random_config(seed):
-    groups, cutoff, samples, step, form, nodes, tile, budget, cull, vw = draw(seed)
-    desc = (f"seed {seed}: {[(g[3], g[0].shape[0]) for g in groups]} cutoff {cutoff} samples {samples} "
+@pytest.mark.parametrize("seed,phi", [(s, "analytic") for s in range(48)] + [(s, "reference") for s in range(200, 216)])
+def test_random_config(seed, phi):
+    """phi: the exact transforms, or (SPH and Wendland groups only: the reference's Gaussian
+    exp(+pi r^2) overflows at these radii) the reference's formulas with their quadratic fits."""
+    bases = BASES if phi == "analytic" else ["SPH", "WENDLAND_D3_C2"]
+    groups, cutoff, samples, step, form, nodes, tile, budget, cull, vw = draw(seed, bases)
+    desc = (f"seed {seed} {phi}: {[(g[3], g[0].shape[0]) for g in groups]} cutoff {cutoff} samples {samples} "
             f"form {form} nodes {nodes} tile {tile} budget {budget} cull {cull}")
     print(f"[case] {desc}", flush=True)
     ds = host.Dataset(groups)
     try:
         abi.lib().vis_set_partial_budget(budget)
         with host.Config(cutoff=cutoff, samples=samples, step=step, u=vw.u, v=vw.v, form=form,
-                         phi_mode="analytic", nodes=nodes, tile=tile, block_length=64, cull=cull) as cfg:
+                         phi_mode=phi, nodes=nodes, tile=tile, block_length=64, cull=cull) as cfg:
             cfg.register(ds)
             try:
                 cfg.render(ds)
@@ -99,7 +102,7 @@ def test_random_config(seed):
     finally:
         abi.lib().vis_set_partial_budget(0)
     ref_groups = culled(groups, step, vw) if cull else groups
-    F64 = oracle.spectrum64(ref_groups, cutoff, step, vw.u, vw.v, "analytic")
+    F64 = oracle.spectrum64(ref_groups, cutoff, step, vw.u, vw.v, phi)
     e = oracle.rel_l2(F, F64)
     ref = np.fft.irfft2(oracle.arrange64(F.astype(np.complex128), *cutoff, *samples), s=samples, norm="forward")
     e_img = oracle.rel_l2(img, ref)
