@@ -404,14 +404,15 @@ static void launch_sum_kind(int grid, hipStream_t s, const MfmaJob* jobs, const 
 // instead of 181 VALU instructions, but the producers are not all of an item's time).
 constexpr double kZeroColExtra = 0.10;
 
+// CUs of the device current at the first call (thread-safe static initialisation).
 static int device_cus() {
-  static int cus = 0;
-  if (!cus) {
-    int dev = 0;
+  static const int cus = [] {
+    int dev = 0, n = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-      cus = 256;
-  }
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
+      n = 256;
+    return n;
+  }();
   return cus;
 }
 
