@@ -226,7 +226,9 @@ void vis_fourier_volume_rendering_noalloc(MeshlessDataset* meshless_dataset, Vis
 double vis_autotune(MeshlessDataset* meshless_dataset, VisConfig* vis_config, double tolerance);
 
 /* Kernel timing with HIP events on the config's stream around the direct-sum
-   kernel (the dominant kernel of a render).  Off by default. */
+   kernel (the dominant kernel of a render; for an MFMA render split into passes by
+   vis_set_partial_budget, from its first pass's kernels to its last pass's, with the
+   reductions between them).  Off by default. */
 void vis_set_kernel_timing(VisConfig* vis_config, int enable);
 /* Returns the number of timed launches since the last reset and their summed
    duration in milliseconds (synchronises the config's stream). */
